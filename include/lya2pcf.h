@@ -1,0 +1,173 @@
+/*
+ * lya2pcf MI355X C ABI.
+ *
+ * Plain C, plain pointers and sizes.  Device buffers are caller-owned
+ * (allocated by the caller, e.g. as PyTorch-ROCm tensors) and passed as
+ * pointers; the library allocates no output.  Every function returns 0 on
+ * success or a negative LYA_ERR_* code, with a message in lya_last_error().
+ * No C++ exception crosses this boundary.  `stream` is a hipStream_t passed
+ * as void* (NULL = the default stream).  Calls on one lya_xi_problem are not
+ * thread-safe.
+ *
+ * Reference interfaces replaced (Rafael9104/lya2pcf):
+ *   lya_build_pairs     <- quasar.neighborhood          forest_class.py:78-98
+ *   lya_build_work      <- per-forest1 launch loop      correlation_procedures_pycuda.py:141-166
+ *   lya_xi_run          <- pair_correlation kernel      cuda_kernels.cpp:90-162, launched
+ *                          once per forest1 with grid (n1,1,1), block (1,16,16)
+ *                          (correlation_procedures_pycuda.py:157-163)
+ *   lya_xi_graph_*      <- the same launch sequence, captured once
+ *   lya_xi_adjoint      <- (no counterpart: gradient of the pair sums)
+ *   lya_distortion_run  <- precompute_distances/compute_etas/order_active/compute_d
+ *                          cuda_kernels.cpp:46-87,165-278 driven by
+ *                          distortion_procedures_pycuda.py:195-279
+ */
+#ifndef LYA2PCF_H
+#define LYA2PCF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LYA_OK 0
+#define LYA_ERR_ARG (-1)
+#define LYA_ERR_HIP (-2)
+#define LYA_ERR_CAPACITY (-3)
+#define LYA_ERR_UNSUPPORTED (-4)
+
+#define LYA_MODE_FP64 0   /* plain FP64 pair loop, FP64 LDS histograms */
+#define LYA_MODE_MIXED 1  /* FP32 geometry + FP64 near-edge recheck, FP32 LDS -> FP64 merge */
+
+/* Largest (numpix_rp * row stride) the LDS histogram holds. */
+#define LYA_MAX_BINS 2560
+
+const char* lya_last_error(void);
+int lya_version(void);
+
+/*
+ * Forest-pair list (host, OpenMP).  Forest f2 is a neighbour of f1 when
+ *   (x1*x2 + y1*y2) + z1*z2 > cos_max   (FP64, no fused multiply-add),
+ *   name1 != name2  and  ra1 < ra2,
+ * the rule of forest_class.py:96.  Output is CSR over f1: neighbours of f1
+ * are nb_idx[nb_off[f1] .. nb_off[f1+1]), in ascending f2 order.
+ * Call with nb_idx == NULL to get the counts (nb_off filled, *n_pairs set);
+ * then call again with nb_idx of capacity >= *n_pairs.
+ */
+int lya_build_pairs(int32_t n_forest, const double* ux, const double* uy, const double* uz,
+                    const double* ra, const int64_t* name, double cos_max,
+                    int64_t* nb_off, int32_t* nb_idx, int64_t nb_capacity, int64_t* n_pairs);
+
+/*
+ * Work decomposition (host).  Forests f1 listed in f1_list (grouped by
+ * output slot f1_slot) are cut into items {f1, nb_begin, nb_end, slot} of at
+ * most item_cost pixel pairs (at least one neighbour each); consecutive items
+ * of one slot are grouped into tiles {item_begin, item_end} of at most
+ * tile_cost, and tiles are ordered by decreasing cost.
+ * Call with items == NULL to get *n_items and *n_tiles; then with buffers of
+ * 4*n_items and 2*n_tiles int32.  *total_cost = sum of n1*n2 over pairs.
+ */
+int lya_build_work(int32_t n_f1, const int32_t* f1_list, const int32_t* f1_slot,
+                   const int64_t* pix_off, const int64_t* nb_off, const int32_t* nb_idx,
+                   int64_t item_cost, int64_t tile_cost,
+                   int32_t* items, int32_t* tiles, int32_t* n_items, int32_t* n_tiles,
+                   double* total_cost);
+
+/* One xi problem: all pointers are device pointers except where noted. */
+typedef struct lya_xi_problem {
+    int32_t n_forest;
+    int32_t n_slots;
+    const int64_t* pix_off;   /* [n_forest+1] pixel offsets of each forest            */
+    const double* ux;         /* [n_forest] unit vector of the line of sight          */
+    const double* uy;
+    const double* uz;
+    const double* base;       /* [n_forest] integer-valued origin of off32 (mixed)    */
+    const uint8_t* sorted;    /* [n_forest] 1 if dc increases along the forest        */
+    const double* dc;         /* [n_pix] comoving distance, FP64                      */
+    const double* we;         /* [n_pix] weight, FP64 (FP64 mode)                     */
+    const double* dw;         /* [n_pix] weight*delta, FP64 (FP64 mode)               */
+    const float* off32;       /* [n_pix] dc - base[forest], FP32 (mixed)              */
+    const float* we32;        /* [n_pix] weight, FP32 (mixed)                         */
+    const float* dw32;        /* [n_pix] weight*delta, FP32 (mixed)                   */
+    const int32_t* nb_idx;    /* [n_pairs] neighbour forest indices                   */
+    const int32_t* items;     /* [n_items][4] f1, nb_begin, nb_end, slot              */
+    const int32_t* tiles;     /* [n_tiles][2] item_begin, item_end                    */
+    int32_t n_items;
+    int32_t n_tiles;
+    int32_t np_rp;            /* bins along r_parallel                                */
+    int32_t np_rt;            /* bins along r_perp                                    */
+    double rpmax;
+    double rtmax;
+    float tau;                /* mixed: half-width of the FP64 recheck band, bin units */
+    int32_t pad0;
+    uint64_t* out_cnt;        /* [n_slots][np_rp*np_rt] pair counts                   */
+    double* out_w;            /* [n_slots][np_rp*np_rt] sum w1*w2                     */
+    double* out_dw;           /* [n_slots][np_rp*np_rt] sum dw1*dw2                   */
+    uint32_t* queue;          /* [>= 4] device scratch: work-queue counter            */
+} lya_xi_problem;
+
+/* Kernel shape, chosen by the autotuner.  Zero fields take defaults. */
+typedef struct lya_xi_config {
+    int32_t mode;             /* LYA_MODE_FP64 or LYA_MODE_MIXED                      */
+    int32_t px_per_lane;      /* 1, 2, 4: forest-1 tile of 64, 128, 256 pixels        */
+    int32_t waves;            /* waves per workgroup: 4 or 8                          */
+    int32_t replicas;         /* LDS histogram copies: 1, 2, 4                        */
+    int32_t row_pad;          /* extra LDS columns per r_parallel row: 0 or 1         */
+    int32_t groups;           /* workgroups in the persistent grid (0 = fill the GPU)  */
+} lya_xi_config;
+
+/* Largest mixed-mode error bound for these data (host): returns tau in bin units. */
+int lya_xi_tau(const lya_xi_problem* host_view_of_sizes, double max_abs_off, double max_c_binner,
+               double max_s_binner, double* tau);
+
+/* Zero outputs and queue, then launch the persistent kernel on `stream`. */
+int lya_xi_run(const lya_xi_problem* p, const lya_xi_config* cfg, void* stream);
+
+/* The same sequence captured into a hipGraph (zeroing uses fill kernels). */
+typedef struct lya_graph_s* lya_graph_t;
+int lya_xi_graph_create(const lya_xi_problem* p, const lya_xi_config* cfg, void* stream,
+                        lya_graph_t* out);
+int lya_graph_launch(lya_graph_t g, void* stream);
+int lya_graph_destroy(lya_graph_t g);
+
+/* Workgroups the persistent grid uses for cfg (0 on error). */
+int lya_xi_grid(const lya_xi_config* cfg, int32_t* groups);
+
+/*
+ * Adjoint of the pair sums.  For every pair (i in forest f1, j in neighbour
+ * f2) in bin A:  g_dw[i] += alpha[A] * dw[j],  g_w[i] += beta[A] * w[j]
+ * (alpha, beta: [np_rp*np_rt] FP64, global bins).  Gradients w.r.t. the
+ * forest-2 side come from a second call with the transposed pair list.
+ * Uses the mixed geometry (exact bins) when cfg->mode is LYA_MODE_MIXED.
+ * g_dw, g_w: [n_pix] FP64, accumulated (caller zeroes).
+ */
+int lya_xi_adjoint(const lya_xi_problem* p, const lya_xi_config* cfg, const double* alpha,
+                   const double* beta, double* g_dw, double* g_w, void* stream);
+
+/*
+ * Distortion matrix, FP64 accumulation.  Work items are forest pairs
+ * {f1, f2} (already subsampled).  Per pair, with pixel bins A(i,j):
+ *   D[A][B] += w_i w_j (eta terms of cuda_kernels.cpp:263-272) for every B
+ *   active in the pair, D[A][A] += w_i w_j, wA[A] += w_i w_j.
+ * dl = delta_lambda, omega = sum w, odl2 = sum w dl^2 per forest (FP64).
+ * D: [nb][nb] FP64 with nb = np_rp*np_rt, wA: [nb]; both accumulated.
+ * scratch: device buffer of at least lya_distortion_scratch_bytes() bytes.
+ */
+int64_t lya_distortion_scratch_bytes(int32_t groups, int32_t max_len, int32_t nbins);
+int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, const double* dc, const double* we,
+                       const double* dl, const double* ux, const double* uy, const double* uz,
+                       const double* omega, const double* odl2, const int32_t* pairs, int32_t n_pairs,
+                       int32_t max_len, int32_t np_rp, int32_t np_rt, double rpmax, double rtmax,
+                       double* D, double* wA, void* scratch, int64_t scratch_bytes, int32_t groups,
+                       void* stream);
+
+/* Test helpers. */
+/* c = sqrt(0.5*(1+cos12)), s = sqrt(0.5*(1-cos12)) evaluated by the device code path. */
+int lya_debug_halfangles(const double* cos12, double* c, double* s, int32_t n, void* stream);
+/* Plain device copy of n bytes (profiler calibration of HBM byte counters). */
+int lya_copy_bytes(const void* src, void* dst, int64_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,761 @@
+// Pair-binning kernels for gfx950 (MI355X).
+//
+// For every forest pair (f1, f2) of the neighbour list and every pixel pair
+// (i in f1, j in f2) the reference (cuda_kernels.cpp:90-162) evaluates
+//     c  = sqrt(0.5 (1 + cos12)),  s = sqrt(0.5 (1 - cos12)),
+//     rp = |r1 - r2| c,  rt = (r1 + r2) s,
+//     binp = floor(rp * NP / rpmax),  bint = floor(rt * NT / rtmax)
+// and, when both bins are in range, adds w1 w2 and dw1 dw2 to the bin.
+//
+// Execution model (one persistent grid):
+//   * work = tiles (runs of items of one output slot, taken from an atomic
+//     queue); item = (forest1, a range of its neighbours); task = (64*PPL-pixel
+//     slab of forest1, one neighbour).  A wave runs one task at a time.
+//   * lanes hold forest-1 pixels in registers; forest-2 pixels are walked in a
+//     wave-uniform loop and read with scalar loads (constant address space),
+//     so every lane sees the same j and no LDS traffic is spent on inputs.
+//   * per-task j range is pruned to pixels that can fall inside rmax for some
+//     lane (forests are monotone in dc); pruned pairs are provably out of range.
+//   * histograms are privatised in LDS (one copy per workgroup, optionally
+//     replicated) and merged into per-slot FP64 global sums with atomics at the
+//     end of each tile.
+//
+// Precision modes:
+//   LYA_MODE_FP64   the reference formula in FP64 (no FMA contraction),
+//                   FP64 LDS sums.
+//   LYA_MODE_MIXED  FP32 geometry on per-forest offsets (dc - base), binner
+//                   folded into the half-angles, FP32 products and FP32 LDS
+//                   partials flushed every item into FP64 LDS sums; any pixel
+//                   pair whose scaled rp or rt lies within the rounding bound
+//                   tau of a bin edge is recomputed with the FP64 formula, so
+//                   the bin indices equal the FP64 ones exactly.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <type_traits>
+
+#include "../../include/lya2pcf.h"
+#include "errors.h"
+
+#pragma clang fp contract(off)
+
+#define CAS __attribute__((address_space(4)))
+
+namespace {
+
+constexpr int HS = LYA_MAX_BINS;  // LDS histogram entries per replica
+
+template <class T>
+__device__ __forceinline__ const CAS T* cptr(const T* p) {
+    return (const CAS T*)(p);
+}
+
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float unif(float x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ double unid(double x) {
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Half-angle factors of a forest pair, the reference expression evaluated in
+// FP64 without contraction.  cos12 is clamped to [-1, 1] (the reference would
+// produce NaN for cos12 > 1, which only happens for coincident lines of sight).
+__device__ __forceinline__ void halfangles(double x1, double y1, double z1, double x2, double y2, double z2,
+                                           double& c, double& s) {
+    double cos12 = x1 * x2 + y1 * y2;
+    cos12 = cos12 + z1 * z2;
+    cos12 = fmin(1.0, fmax(-1.0, cos12));
+    c = sqrt(0.5 * (1.0 + cos12));
+    s = sqrt(0.5 * (1.0 - cos12));
+}
+
+template <class T>
+__device__ __forceinline__ void lds_add(unsigned char* lds, uint32_t byte_off, T v) {
+    __hip_atomic_fetch_add((T*)(lds + byte_off), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// LDS layout (bytes).  MIXED: per replica [cnt u32 | w f32 | dw f32] x HS, then
+// the FP64 accumulators [w f64 | dw f64] x HS.  FP64: per replica
+// [cnt u32 | w f64 | dw f64] x HS.
+template <int MODE>
+struct Layout {
+    static constexpr uint32_t REPL = (MODE == LYA_MODE_MIXED) ? 12u * HS : 20u * HS;
+    static constexpr uint32_t W = 4u * HS;                                      // w offset in a replica
+    static constexpr uint32_t DW = (MODE == LYA_MODE_MIXED) ? 8u * HS : 12u * HS;  // dw offset
+    static __host__ __device__ uint32_t bytes(int R) {
+        return REPL * (uint32_t)R + ((MODE == LYA_MODE_MIXED) ? 16u * HS : 0u);
+    }
+    static __host__ __device__ uint32_t acc(int R) { return REPL * (uint32_t)R; }
+};
+
+struct TaskCtx {
+    int f1;
+    int64_t p1;
+    int n1;
+    int slot;
+};
+
+// Forest-2 j range that can hold an in-range pair for some forest-1 pixel of the
+// slab.  Needs both forests sorted by dc; the margins (1e-3 bin) dominate the
+// FP64 rounding of the thresholds, so skipped pairs are out of range exactly.
+__device__ __forceinline__ void prune(const lya_xi_problem& p, int f1, int64_t p1, int i0, int ilast, int f2,
+                                      int64_t p2, int n2, double c, double s, double bp64, double bt64, int lane,
+                                      int& jlo, int& jhi) {
+    jlo = 0;
+    jhi = n2;
+    if (!(cptr(p.sorted)[f1] & cptr(p.sorted)[f2])) return;
+    const double r1min = cptr(p.dc)[p1 + i0];
+    const double r1max = cptr(p.dc)[p1 + ilast];
+    const double Dp = ((double)p.np_rp + 1e-3) / (c * bp64);
+    const double Rt = ((double)p.np_rt + 1e-3) / (s * bt64);
+    const double lo = r1min - Dp;
+    const double hi = fmin(r1max + Dp, Rt - r1min);
+    int clo = 0, chi = 0;
+    const int nr = (n2 + 63) & ~63;
+    for (int jj = lane; jj < nr; jj += 64) {
+        const double r2 = (jj < n2) ? p.dc[p2 + jj] : INFINITY;
+        clo += __popcll(__ballot(r2 <= lo));
+        chi += __popcll(__ballot(r2 < hi));
+    }
+    jlo = uni(clo);
+    jhi = uni(chi > clo ? chi : clo);
+}
+
+// ---------------------------------------------------------------------------
+// One task: a 64*PPL-pixel slab of forest 1 against all pixels of forest 2.
+// ---------------------------------------------------------------------------
+template <int MODE, int PPL>
+__device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char* lds, uint32_t hbase,
+                                         uint32_t ntpad4, const TaskCtx& t, int slab, int f2, int lane, int perm,
+                                         double bp64, double bt64) {
+    using L = Layout<MODE>;
+    const int NP = p.np_rp, NT = p.np_rt;
+    const int64_t p2 = cptr(p.pix_off)[f2];
+    const int n2 = (int)(cptr(p.pix_off)[f2 + 1] - p2);
+    double c, s;
+    halfangles(cptr(p.ux)[t.f1], cptr(p.uy)[t.f1], cptr(p.uz)[t.f1], cptr(p.ux)[f2], cptr(p.uy)[f2],
+               cptr(p.uz)[f2], c, s);
+    c = unid(c);
+    s = unid(s);
+    const int i0 = slab * 64 * PPL;
+    const int ilast = min(t.n1, i0 + 64 * PPL) - 1;
+    int jlo, jhi;
+    prune(p, t.f1, t.p1, i0, ilast, f2, p2, n2, c, s, bp64, bt64, lane, jlo, jhi);
+    if (jlo >= jhi) return;
+
+    double r1[PPL];
+    int64_t pix1[PPL];
+    bool valid[PPL];
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        const int i = i0 + k * 64 + perm;
+        valid[k] = i < t.n1;
+        pix1[k] = t.p1 + i;
+        r1[k] = valid[k] ? p.dc[pix1[k]] : 1.0e9;
+    }
+
+    if constexpr (MODE == LYA_MODE_FP64) {
+        double w1[PPL], d1[PPL];
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) {
+            w1[k] = valid[k] ? p.we[pix1[k]] : 0.0;
+            d1[k] = valid[k] ? p.dw[pix1[k]] : 0.0;
+        }
+        const CAS double* r2p = cptr(p.dc) + p2;
+        const CAS double* w2p = cptr(p.we) + p2;
+        const CAS double* d2p = cptr(p.dw) + p2;
+        auto block = [&](int j, auto UC) {
+            constexpr int UU = decltype(UC)::value;
+            double r2[UU], w2[UU], d2[UU];
+#pragma unroll
+            for (int u = 0; u < UU; ++u) {
+                r2[u] = r2p[j + u];
+                w2[u] = w2p[j + u];
+                d2[u] = d2p[j + u];
+            }
+#pragma unroll
+            for (int u = 0; u < UU; ++u) {
+#pragma unroll
+                for (int k = 0; k < PPL; ++k) {
+                    const double rp = fabs(r1[k] - r2[u]) * c;
+                    const double rt = (r1[k] + r2[u]) * s;
+                    const uint32_t bp = (uint32_t)__double2int_rz(rp * bp64);
+                    const uint32_t bt = (uint32_t)__double2int_rz(rt * bt64);
+                    const double pw = w1[k] * w2[u], pd = d1[k] * d2[u];
+                    if (bp < (uint32_t)NP && bt < (uint32_t)NT) {
+                        const uint32_t a4 = __umul24(bp, ntpad4) + (bt << 2);
+                        lds_add<uint32_t>(lds, hbase + a4, 1u);
+                        lds_add<double>(lds, hbase + L::W + 2u * a4, pw);
+                        lds_add<double>(lds, hbase + L::DW + 2u * a4, pd);
+                    }
+                }
+            }
+        };
+        constexpr int U = 4;
+        int j = jlo;
+        for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
+        for (; j < jhi; ++j) block(j, std::integral_constant<int, 1>{});
+    } else {
+        const double base2 = cptr(p.base)[f2];
+        const double cb = c * bp64, sb = s * bt64;
+        const float c4 = unif((float)cb);
+        const float nc4 = -c4;
+        const float s4 = unif((float)sb);
+        const float tau = p.tau;
+        const float tau2 = 2.0f * tau;
+        float P1[PPL], T1[PPL], w1[PPL], d1[PPL];
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) {
+            if (valid[k]) {
+                P1[k] = (float)((r1[k] - base2) * cb);
+                T1[k] = (float)((r1[k] + base2) * sb + (double)tau);
+                w1[k] = p.we32[pix1[k]];
+                d1[k] = p.dw32[pix1[k]];
+            } else {
+                P1[k] = 1048576.5f;  // 2^20 + 1/2: far out of range, fract = 1/2
+                T1[k] = 1048576.5f;
+                w1[k] = 0.f;
+                d1[k] = 0.f;
+            }
+        }
+        const CAS float* o2p = cptr(p.off32) + p2;
+        const CAS float* w2p = cptr(p.we32) + p2;
+        const CAS float* d2p = cptr(p.dw32) + p2;
+        const CAS double* r2p = cptr(p.dc) + p2;
+        constexpr int U = 8;
+        auto block = [&](int j, auto UC) {
+            constexpr int UU = decltype(UC)::value;
+            uint32_t bp[UU][PPL], bt[UU][PPL];
+            float fr[UU][PPL];
+            float o2v[UU], w2v[UU], d2v[UU];
+            float frmin = 1.0f;
+#pragma unroll
+            for (int u = 0; u < UU; ++u) {
+                o2v[u] = o2p[j + u];
+                w2v[u] = w2p[j + u];
+                d2v[u] = d2p[j + u];
+            }
+#pragma unroll
+            for (int u = 0; u < UU; ++u) {
+                const float o2 = o2v[u];
+#pragma unroll
+                for (int k = 0; k < PPL; ++k) {
+                    const float v = __builtin_fmaf(o2, nc4, P1[k]);
+                    const float qp = __builtin_fabsf(v) + tau;
+                    const float qt = __builtin_fmaf(o2, s4, T1[k]);
+                    bp[u][k] = __float2uint_rz(qp);
+                    bt[u][k] = __float2uint_rz(qt);
+                    fr[u][k] = fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt));
+                    frmin = fminf(frmin, fr[u][k]);
+                }
+            }
+            if (frmin < tau2) {
+                // near a bin edge somewhere in the block: exact FP64 bins for those pairs
+#pragma unroll
+                for (int u = 0; u < UU; ++u) {
+#pragma unroll
+                    for (int k = 0; k < PPL; ++k) {
+                        if (fr[u][k] < tau2) {
+                            const double r2 = r2p[j + u];
+                            const double rp = fabs(r1[k] - r2) * c;
+                            const double rt = (r1[k] + r2) * s;
+                            bp[u][k] = (uint32_t)__double2int_rz(rp * bp64);
+                            bt[u][k] = (uint32_t)__double2int_rz(rt * bt64);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UU; ++u) {
+#pragma unroll
+                for (int k = 0; k < PPL; ++k) {
+                    const float pw = w1[k] * w2v[u], pd = d1[k] * d2v[u];
+                    if (bp[u][k] < (uint32_t)NP && bt[u][k] < (uint32_t)NT) {
+                        const uint32_t a = __umul24(bp[u][k], ntpad4) + (bt[u][k] << 2) + hbase;
+                        lds_add<uint32_t>(lds, a, 1u);
+                        lds_add<float>(lds, a + L::W, pw);
+                        lds_add<float>(lds, a + L::DW, pd);
+                    }
+                }
+            }
+        };
+        int j = jlo;
+        for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
+        for (; j < jhi; ++j) block(j, std::integral_constant<int, 1>{});
+    }
+}
+
+// Mixed mode: move the FP32 partial sums into the FP64 accumulators.
+__device__ void flush_item(unsigned char* lds, int R, int nb_lds, int tid, int nthr) {
+    using L = Layout<LYA_MODE_MIXED>;
+    double* aw = (double*)(lds + L::acc(R));
+    double* ad = aw + HS;
+    for (int b = tid; b < nb_lds; b += nthr) {
+        double sw = 0.0, sd = 0.0;
+        for (int r = 0; r < R; ++r) {
+            float* hw = (float*)(lds + r * L::REPL + L::W);
+            float* hd = (float*)(lds + r * L::REPL + L::DW);
+            sw += (double)hw[b];
+            sd += (double)hd[b];
+            hw[b] = 0.f;
+            hd[b] = 0.f;
+        }
+        aw[b] += sw;
+        ad[b] += sd;
+    }
+}
+
+// End of tile: merge the workgroup's sums into the slot's FP64 global sums.
+template <int MODE>
+__device__ void flush_tile(const lya_xi_problem& p, unsigned char* lds, int R, int ntpad, int slot, int tid,
+                           int nthr) {
+    using L = Layout<MODE>;
+    const int nb_lds = p.np_rp * ntpad;
+    const int64_t gofs = (int64_t)slot * p.np_rp * p.np_rt;
+    for (int b = tid; b < nb_lds; b += nthr) {
+        uint32_t cnt = 0;
+        double sw = 0.0, sd = 0.0;
+        for (int r = 0; r < R; ++r) {
+            uint32_t* hc = (uint32_t*)(lds + r * L::REPL);
+            cnt += hc[b];
+            hc[b] = 0u;
+            if constexpr (MODE == LYA_MODE_FP64) {
+                double* hw = (double*)(lds + r * L::REPL + L::W);
+                double* hd = (double*)(lds + r * L::REPL + L::DW);
+                sw += hw[b];
+                sd += hd[b];
+                hw[b] = 0.0;
+                hd[b] = 0.0;
+            }
+        }
+        if constexpr (MODE == LYA_MODE_MIXED) {
+            double* aw = (double*)(lds + L::acc(R));
+            double* ad = aw + HS;
+            sw = aw[b];
+            sd = ad[b];
+            aw[b] = 0.0;
+            ad[b] = 0.0;
+        }
+        if (cnt) {
+            const int bp = b / ntpad, bt = b - bp * ntpad;
+            const int64_t g = gofs + (int64_t)bp * p.np_rt + bt;
+            atomicAdd((unsigned long long*)(p.out_cnt + g), (unsigned long long)cnt);
+            unsafeAtomicAdd(p.out_w + g, sw);
+            unsafeAtomicAdd(p.out_dw + g, sd);
+        }
+    }
+}
+
+template <int MODE, int PPL, int WAVES>
+__global__ void __launch_bounds__(WAVES * 64) xi_kernel(const lya_xi_problem p, int R, int ntpad) {
+    using L = Layout<MODE>;
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int s_tile;
+    constexpr int NTHR = WAVES * 64;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = uni(tid >> 6);
+    const int perm = ((lane & 31) << 1) | (lane >> 5);
+    const double bp64 = (double)p.np_rp / p.rpmax;
+    const double bt64 = (double)p.np_rt / p.rtmax;
+    const uint32_t ntpad4 = (uint32_t)ntpad * 4u;
+    const uint32_t hbase = (uint32_t)(wave % R) * L::REPL;
+    const int nb_lds = p.np_rp * ntpad;
+
+    const uint32_t nbytes = L::bytes(R);
+    for (uint32_t o = (uint32_t)tid * 16u; o < nbytes; o += NTHR * 16u)
+        *(uint4*)(lds + o) = make_uint4(0u, 0u, 0u, 0u);
+    __syncthreads();
+
+    for (;;) {
+        if (tid == 0) s_tile = (int)atomicAdd(p.queue, 1u);
+        __syncthreads();
+        const int tile = uni(s_tile);
+        if (tile >= p.n_tiles) break;
+        const int ib = cptr(p.tiles)[2 * tile], ie = cptr(p.tiles)[2 * tile + 1];
+        const int slot = cptr(p.items)[4 * ib + 3];
+        for (int it = ib; it < ie; ++it) {
+            TaskCtx t;
+            t.f1 = cptr(p.items)[4 * it + 0];
+            const int nb0 = cptr(p.items)[4 * it + 1];
+            const int nb1 = cptr(p.items)[4 * it + 2];
+            t.slot = slot;
+            t.p1 = cptr(p.pix_off)[t.f1];
+            t.n1 = (int)(cptr(p.pix_off)[t.f1 + 1] - t.p1);
+            const int nslab = (t.n1 + 64 * PPL - 1) / (64 * PPL);
+            const int ntask = nslab * (nb1 - nb0);
+            for (int k = wave; k < ntask; k += WAVES) {
+                const int q = k / nslab;
+                const int slab = k - q * nslab;
+                const int f2 = cptr(p.nb_idx)[nb0 + q];
+                run_task<MODE, PPL>(p, lds, hbase, ntpad4, t, slab, f2, lane, perm, bp64, bt64);
+            }
+            if constexpr (MODE == LYA_MODE_MIXED) {
+                __syncthreads();
+                flush_item(lds, R, nb_lds, tid, NTHR);
+            }
+        }
+        __syncthreads();
+        flush_tile<MODE>(p, lds, R, ntpad, slot, tid, NTHR);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint: g_dw[i] += alpha[A] dw[j], g_w[i] += beta[A] w[j] over the pairs of
+// the list, lanes = forest-1 pixels (no atomics in the pair loop).
+// ---------------------------------------------------------------------------
+template <int MODE, int WAVES>
+__global__ void __launch_bounds__(WAVES * 64) xi_adjoint_kernel(const lya_xi_problem p, const double* alpha,
+                                                                const double* beta, double* g_dw, double* g_w) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int s_tile;
+    constexpr int NTHR = WAVES * 64;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = uni(tid >> 6);
+    const int NP = p.np_rp, NT = p.np_rt;
+    const int nb = NP * NT;
+    double* ta = (double*)lds;
+    double* tb = ta + nb;
+    for (int b = tid; b < nb; b += NTHR) {
+        ta[b] = alpha[b];
+        tb[b] = beta[b];
+    }
+    __syncthreads();
+    const double bp64 = (double)NP / p.rpmax;
+    const double bt64 = (double)NT / p.rtmax;
+
+    for (;;) {
+        if (tid == 0) s_tile = (int)atomicAdd(p.queue, 1u);
+        __syncthreads();
+        const int tile = uni(s_tile);
+        __syncthreads();
+        if (tile >= p.n_tiles) break;
+        const int ib = cptr(p.tiles)[2 * tile], ie = cptr(p.tiles)[2 * tile + 1];
+        for (int it = ib; it < ie; ++it) {
+            const int f1 = cptr(p.items)[4 * it + 0];
+            const int nb0 = cptr(p.items)[4 * it + 1];
+            const int nb1 = cptr(p.items)[4 * it + 2];
+            const int64_t p1 = cptr(p.pix_off)[f1];
+            const int n1 = (int)(cptr(p.pix_off)[f1 + 1] - p1);
+            const int nslab = (n1 + 63) / 64;
+            const int ntask = nslab * (nb1 - nb0);
+            for (int k = wave; k < ntask; k += WAVES) {
+                const int q = k / nslab;
+                const int slab = k - q * nslab;
+                const int f2 = cptr(p.nb_idx)[nb0 + q];
+                const int64_t p2 = cptr(p.pix_off)[f2];
+                const int n2 = (int)(cptr(p.pix_off)[f2 + 1] - p2);
+                double c, s;
+                halfangles(cptr(p.ux)[f1], cptr(p.uy)[f1], cptr(p.uz)[f1], cptr(p.ux)[f2], cptr(p.uy)[f2],
+                           cptr(p.uz)[f2], c, s);
+                c = unid(c);
+                s = unid(s);
+                const int i = slab * 64 + lane;
+                const bool valid = i < n1;
+                const int ilast = min(n1, slab * 64 + 64) - 1;
+                int jlo, jhi;
+                prune(p, f1, p1, slab * 64, ilast, f2, p2, n2, c, s, bp64, bt64, lane, jlo, jhi);
+                if (jlo >= jhi || !valid) continue;
+                const double r1 = p.dc[p1 + i];
+                double gd = 0.0, gw = 0.0;
+                if constexpr (MODE == LYA_MODE_FP64) {
+                    for (int j = jlo; j < jhi; ++j) {
+                        const double r2 = cptr(p.dc)[p2 + j];
+                        const uint32_t bp = (uint32_t)__double2int_rz((fabs(r1 - r2) * c) * bp64);
+                        const uint32_t bt = (uint32_t)__double2int_rz(((r1 + r2) * s) * bt64);
+                        if (bp < (uint32_t)NP && bt < (uint32_t)NT) {
+                            const int A = bp * NT + bt;
+                            gd += ta[A] * cptr(p.dw)[p2 + j];
+                            gw += tb[A] * cptr(p.we)[p2 + j];
+                        }
+                    }
+                } else {
+                    const double base2 = cptr(p.base)[f2];
+                    const double cb = c * bp64, sb = s * bt64;
+                    const float c4 = unif((float)cb), s4 = unif((float)sb);
+                    const float tau = p.tau, tau2 = 2.0f * tau;
+                    const float P1 = (float)((r1 - base2) * cb);
+                    const float T1 = (float)((r1 + base2) * sb + (double)tau);
+                    for (int j = jlo; j < jhi; ++j) {
+                        const float o2 = cptr(p.off32)[p2 + j];
+                        const float qp = __builtin_fabsf(__builtin_fmaf(o2, -c4, P1)) + tau;
+                        const float qt = __builtin_fmaf(o2, s4, T1);
+                        uint32_t bp = __float2uint_rz(qp), bt = __float2uint_rz(qt);
+                        if (fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt)) < tau2) {
+                            const double r2 = cptr(p.dc)[p2 + j];
+                            bp = (uint32_t)__double2int_rz((fabs(r1 - r2) * c) * bp64);
+                            bt = (uint32_t)__double2int_rz(((r1 + r2) * s) * bt64);
+                        }
+                        if (bp < (uint32_t)NP && bt < (uint32_t)NT) {
+                            const int A = bp * NT + bt;
+                            gd += ta[A] * (double)cptr(p.dw32)[p2 + j];
+                            gw += tb[A] * (double)cptr(p.we32)[p2 + j];
+                        }
+                    }
+                }
+                unsafeAtomicAdd(g_dw + p1 + i, gd);
+                unsafeAtomicAdd(g_w + p1 + i, gw);
+            }
+        }
+    }
+}
+
+__global__ void zero_outputs(uint64_t* cnt, double* w, double* dw, int64_t n, uint32_t* queue) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (cnt) cnt[i] = 0ull;
+        if (w) w[i] = 0.0;
+        if (dw) dw[i] = 0.0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 4) queue[threadIdx.x] = 0u;
+}
+
+__global__ void halfangle_kernel(const double* cos12, double* c, double* s, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // same expression as halfangles() with the dot product replaced by its value
+    double x = fmin(1.0, fmax(-1.0, cos12[i]));
+    c[i] = sqrt(0.5 * (1.0 + x));
+    s[i] = sqrt(0.5 * (1.0 - x));
+}
+
+__global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+using XiKernel = void (*)(const lya_xi_problem, int, int);
+
+template <int MODE, int PPL>
+XiKernel pick_w(int waves) {
+    if (waves == 4) return xi_kernel<MODE, PPL, 4>;
+    if (waves == 8) return xi_kernel<MODE, PPL, 8>;
+    return nullptr;
+}
+template <int MODE>
+XiKernel pick_p(int ppl, int waves) {
+    if (ppl == 1) return pick_w<MODE, 1>(waves);
+    if (ppl == 2) return pick_w<MODE, 2>(waves);
+    if (ppl == 4) return pick_w<MODE, 4>(waves);
+    return nullptr;
+}
+
+struct Resolved {
+    XiKernel fn;
+    int mode, ppl, waves, R, ntpad, groups;
+    uint32_t lds;
+};
+
+int resolve(const lya_xi_problem* p, const lya_xi_config* cfg, Resolved& r) {
+    r.mode = cfg ? cfg->mode : LYA_MODE_MIXED;
+    r.ppl = (cfg && cfg->px_per_lane) ? cfg->px_per_lane : 1;
+    r.waves = (cfg && cfg->waves) ? cfg->waves : 8;
+    r.R = (cfg && cfg->replicas) ? cfg->replicas : 1;
+    const int pad = (cfg && cfg->row_pad > 0) ? cfg->row_pad : 0;
+    if (r.mode != LYA_MODE_FP64 && r.mode != LYA_MODE_MIXED) return lya_fail(LYA_ERR_ARG, "unknown mode %d", r.mode);
+    if (r.R != 1 && r.R != 2 && r.R != 4) return lya_fail(LYA_ERR_ARG, "replicas must be 1, 2 or 4");
+    if (r.R > r.waves) return lya_fail(LYA_ERR_ARG, "replicas > waves");
+    r.fn = (r.mode == LYA_MODE_FP64) ? pick_p<LYA_MODE_FP64>(r.ppl, r.waves) : pick_p<LYA_MODE_MIXED>(r.ppl, r.waves);
+    if (!r.fn) return lya_fail(LYA_ERR_ARG, "unsupported shape px_per_lane=%d waves=%d", r.ppl, r.waves);
+    if (p) {
+        if (p->np_rp < 1 || p->np_rt < 1 || !(p->rpmax > 0) || !(p->rtmax > 0))
+            return lya_fail(LYA_ERR_ARG, "bad binning");
+        r.ntpad = p->np_rt + pad;
+        if ((int64_t)p->np_rp * r.ntpad > HS)
+            return lya_fail(LYA_ERR_UNSUPPORTED, "np_rp*(np_rt+pad)=%d exceeds LDS histogram capacity %d",
+                            p->np_rp * r.ntpad, HS);
+    } else {
+        r.ntpad = 0;
+    }
+    r.lds = (r.mode == LYA_MODE_FP64) ? Layout<LYA_MODE_FP64>::bytes(r.R) : Layout<LYA_MODE_MIXED>::bytes(r.R);
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return lya_fail(LYA_ERR_HIP, "no HIP device");
+    if (r.lds > prop.sharedMemPerBlock)
+        return lya_fail(LYA_ERR_UNSUPPORTED, "LDS %u bytes > %zu per workgroup", r.lds, prop.sharedMemPerBlock);
+    hipFuncSetAttribute((const void*)r.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)r.lds);
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)r.fn, r.waves * 64, r.lds);
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    r.groups = (cfg && cfg->groups > 0) ? cfg->groups : per_cu * prop.multiProcessorCount;
+    return LYA_OK;
+}
+
+int check_problem(const lya_xi_problem* p, int mode) {
+    if (!p) return lya_fail(LYA_ERR_ARG, "null problem");
+    if (!p->pix_off || !p->ux || !p->uy || !p->uz || !p->sorted || !p->dc || !p->nb_idx || !p->items ||
+        !p->tiles || !p->queue)
+        return lya_fail(LYA_ERR_ARG, "null device pointer in problem");
+    if (mode == LYA_MODE_FP64 && (!p->we || !p->dw)) return lya_fail(LYA_ERR_ARG, "FP64 mode needs we, dw");
+    if (mode == LYA_MODE_MIXED && (!p->off32 || !p->we32 || !p->dw32 || !p->base))
+        return lya_fail(LYA_ERR_ARG, "mixed mode needs off32, we32, dw32, base");
+    if (mode == LYA_MODE_MIXED && !(p->tau > 0.0f && p->tau < 0.25f))
+        return lya_fail(LYA_ERR_ARG, "mixed mode needs 0 < tau < 0.25 (got %g)", (double)p->tau);
+    if (p->n_tiles < 0 || p->n_items < 0 || p->n_slots < 0) return lya_fail(LYA_ERR_ARG, "negative sizes");
+    return LYA_OK;
+}
+
+int launch_all(const lya_xi_problem* p, const Resolved& r, hipStream_t st) {
+    const int64_t n = (int64_t)p->n_slots * p->np_rp * p->np_rt;
+    int zb = (int)std::min<int64_t>((n + 255) / 256, 4096);
+    if (zb < 1) zb = 1;
+    hipLaunchKernelGGL(zero_outputs, dim3(zb), dim3(256), 0, st, p->out_cnt, p->out_w, p->out_dw, n, p->queue);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return lya_fail(LYA_ERR_HIP, "zero_outputs launch: %s", hipGetErrorString(e));
+    hipLaunchKernelGGL(r.fn, dim3(r.groups), dim3(r.waves * 64), r.lds, st, *p, r.R, r.ntpad);
+    e = hipGetLastError();
+    if (e != hipSuccess) return lya_fail(LYA_ERR_HIP, "xi kernel launch: %s", hipGetErrorString(e));
+    return LYA_OK;
+}
+
+}  // namespace
+
+struct lya_graph_s {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+extern "C" int lya_xi_grid(const lya_xi_config* cfg, int32_t* groups) {
+    Resolved r;
+    int rc = resolve(nullptr, cfg, r);
+    if (rc) return rc;
+    *groups = r.groups;
+    return LYA_OK;
+}
+
+extern "C" int lya_xi_run(const lya_xi_problem* p, const lya_xi_config* cfg, void* stream) {
+    Resolved r;
+    int rc = resolve(p, cfg, r);
+    if (rc) return rc;
+    if ((rc = check_problem(p, r.mode))) return rc;
+    if (!p->out_cnt || !p->out_w || !p->out_dw) return lya_fail(LYA_ERR_ARG, "null output");
+    return launch_all(p, r, (hipStream_t)stream);
+}
+
+extern "C" int lya_xi_graph_create(const lya_xi_problem* p, const lya_xi_config* cfg, void* stream,
+                                   lya_graph_t* out) {
+    (void)stream;
+    Resolved r;
+    int rc = resolve(p, cfg, r);
+    if (rc) return rc;
+    if ((rc = check_problem(p, r.mode))) return rc;
+    if (!p->out_cnt || !p->out_w || !p->out_dw || !out) return lya_fail(LYA_ERR_ARG, "null output");
+    hipStream_t cs;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
+        return lya_fail(LYA_ERR_HIP, "capture stream create failed");
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        hipStreamDestroy(cs);
+        return lya_fail(LYA_ERR_HIP, "begin capture: %s", hipGetErrorString(e));
+    }
+    rc = launch_all(p, r, cs);
+    e = hipStreamEndCapture(cs, &g);
+    hipStreamDestroy(cs);
+    if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return lya_fail(LYA_ERR_HIP, "end capture: %s", hipGetErrorString(e));
+    hipGraphExec_t x;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(g);
+        return lya_fail(LYA_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    }
+    *out = new lya_graph_s{g, x};
+    return LYA_OK;
+}
+
+extern "C" int lya_graph_launch(lya_graph_t g, void* stream) {
+    if (!g) return lya_fail(LYA_ERR_ARG, "null graph");
+    hipError_t e = hipGraphLaunch(g->exec, (hipStream_t)stream);
+    if (e != hipSuccess) return lya_fail(LYA_ERR_HIP, "graph launch: %s", hipGetErrorString(e));
+    return LYA_OK;
+}
+
+extern "C" int lya_graph_destroy(lya_graph_t g) {
+    if (!g) return LYA_OK;
+    hipGraphExecDestroy(g->exec);
+    hipGraphDestroy(g->graph);
+    delete g;
+    return LYA_OK;
+}
+
+extern "C" int lya_xi_tau(const lya_xi_problem* p, double max_abs_off, double max_c_binner, double max_s_binner,
+                          double* tau) {
+    // Bound on |(FP32 scaled coordinate) - (FP64 scaled coordinate)| for pairs
+    // whose coordinate is below np + 3 bins (larger ones are out of range in
+    // both evaluations).  Terms, each a unit roundoff u32 = 2^-24 times a
+    // magnitude bound, follow the mixed-mode arithmetic of run_task:
+    //   rounding of P1 (or T1), of the fma result, of |v| + tau, the FP32
+    //   representation of off32 and of c*binner (or s*binner), plus the FP64
+    //   roundings of the reference formula.  A factor 2 of headroom is applied.
+    if (!p || !tau || !(max_abs_off >= 0)) return lya_fail(LYA_ERR_ARG, "lya_xi_tau: bad arguments");
+    const double u32 = 1.0 / 16777216.0, u64 = 1.0 / 9007199254740992.0;
+    auto bound = [&](double nb, double kb) {
+        const double q = nb + 3.0;        // largest relevant coordinate
+        const double m = max_abs_off * kb;  // largest |off * k|
+        return u32 * (2.0 * q + (q + m) + 2.0 * m) + 8.0 * u64 * (q + m + 1e4 * kb);
+    };
+    const double t = 2.0 * std::max(bound(p->np_rp, max_c_binner), bound(p->np_rt, max_s_binner));
+    *tau = t;
+    return LYA_OK;
+}
+
+extern "C" int lya_xi_adjoint(const lya_xi_problem* p, const lya_xi_config* cfg, const double* alpha,
+                              const double* beta, double* g_dw, double* g_w, void* stream) {
+    const int mode = cfg ? cfg->mode : LYA_MODE_MIXED;
+    int rc = check_problem(p, mode);
+    if (rc) return rc;
+    if (!alpha || !beta || !g_dw || !g_w) return lya_fail(LYA_ERR_ARG, "null adjoint buffer");
+    const int nb = p->np_rp * p->np_rt;
+    const size_t lds = (size_t)nb * 16;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return lya_fail(LYA_ERR_HIP, "no HIP device");
+    if (lds > prop.sharedMemPerBlock) return lya_fail(LYA_ERR_UNSUPPORTED, "too many bins for the adjoint");
+    auto fn = (mode == LYA_MODE_FP64) ? xi_adjoint_kernel<LYA_MODE_FP64, 8> : xi_adjoint_kernel<LYA_MODE_MIXED, 8>;
+    hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, 512, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    const int groups = (cfg && cfg->groups > 0) ? cfg->groups : per_cu * prop.multiProcessorCount;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(zero_outputs, dim3(1), dim3(64), 0, st, (uint64_t*)nullptr, (double*)nullptr,
+                       (double*)nullptr, (int64_t)0, p->queue);
+    hipLaunchKernelGGL(fn, dim3(groups), dim3(512), lds, st, *p, alpha, beta, g_dw, g_w);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return lya_fail(LYA_ERR_HIP, "adjoint launch: %s", hipGetErrorString(e));
+    return LYA_OK;
+}
+
+extern "C" int lya_debug_halfangles(const double* cos12, double* c, double* s, int32_t n, void* stream) {
+    if (n < 0 || (n > 0 && (!cos12 || !c || !s))) return lya_fail(LYA_ERR_ARG, "bad halfangle arguments");
+    if (n == 0) return LYA_OK;
+    hipLaunchKernelGGL(halfangle_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, cos12, c, s, n);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? LYA_OK : lya_fail(LYA_ERR_HIP, "halfangle launch: %s", hipGetErrorString(e));
+}
+
+extern "C" int lya_copy_bytes(const void* src, void* dst, int64_t n, void* stream) {
+    if (n < 0 || (n % 16) || (n > 0 && (!src || !dst))) return lya_fail(LYA_ERR_ARG, "copy: n must be a multiple of 16");
+    if (n == 0) return LYA_OK;
+    const int64_t n16 = n / 16;
+    const int blocks = (int)std::min<int64_t>((n16 + 255) / 256, 8192);
+    hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (const uint4*)src, (uint4*)dst, n16);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? LYA_OK : lya_fail(LYA_ERR_HIP, "copy launch: %s", hipGetErrorString(e));
+}

@@ -1,0 +1,217 @@
+"""xi(r_par, r_perp) pair histograms on the GPU.
+
+``XiEngine`` owns one problem: the packed forests in HBM, the forest-pair
+list, the work decomposition (items and tiles), the per-slot outputs and the
+work-queue counter.  One call of :meth:`run` (or one replay of
+:meth:`graph`) computes, for every output slot (HEALPix pixel of forest 1),
+  cnt[slot, bp, bt]  number of pixel pairs in the bin,
+  w[slot, bp, bt]    sum of w1 w2,
+  dw[slot, bp, bt]   sum of dw1 dw2,
+with the binning of reference cuda_kernels.cpp:90-162 and the pair list of
+forest_class.py:78-98.  Device buffers are torch tensors (plumbing only).
+"""
+import ctypes as C
+import numpy as np
+
+from . import _lib
+from ._lib import XiProblem, XiConfig, LYA_MODE_FP64, LYA_MODE_MIXED, check, lib, ptr
+
+MODES = {"fp64": LYA_MODE_FP64, "mixed": LYA_MODE_MIXED}
+
+
+def build_pairs(forests, angmax):
+    """CSR neighbour list: (nb_off int64[nf+1], nb_idx int32[n_pairs])."""
+    L = lib()
+    nf = forests.nf
+    nb_off = np.zeros(nf + 1, dtype=np.int64)
+    npairs = C.c_int64(0)
+    ux, uy, uz = (np.ascontiguousarray(a, dtype=np.float64) for a in (forests.ux, forests.uy, forests.uz))
+    ra = np.ascontiguousarray(forests.ra, dtype=np.float64)
+    name = np.ascontiguousarray(forests.name, dtype=np.int64)
+    cosm = float(np.cos(angmax))
+    check(L.lya_build_pairs(nf, ptr(ux), ptr(uy), ptr(uz), ptr(ra), ptr(name), cosm, ptr(nb_off), None, 0,
+                            C.byref(npairs)))
+    nb_idx = np.zeros(max(1, npairs.value), dtype=np.int32)
+    check(L.lya_build_pairs(nf, ptr(ux), ptr(uy), ptr(uz), ptr(ra), ptr(name), cosm, ptr(nb_off), ptr(nb_idx),
+                            len(nb_idx), C.byref(npairs)))
+    return nb_off, nb_idx[:npairs.value]
+
+
+def transpose_pairs(nb_off, nb_idx, nf):
+    """Pair list seen from forest 2: CSR over f2 of the f1 that list it."""
+    counts = np.bincount(nb_idx, minlength=nf).astype(np.int64)
+    off = np.zeros(nf + 1, dtype=np.int64)
+    np.cumsum(counts, out=off[1:])
+    f1 = np.repeat(np.arange(nf, dtype=np.int32), np.diff(nb_off))
+    order = np.argsort(nb_idx, kind="stable")
+    return off, f1[order].astype(np.int32)
+
+
+def build_work(forests, nb_off, nb_idx, f1_list, item_cost, tile_cost):
+    L = lib()
+    f1_list = np.ascontiguousarray(f1_list, dtype=np.int32)
+    f1_slot = np.ascontiguousarray(forests.slot[f1_list], dtype=np.int32)
+    ni, nt = C.c_int32(0), C.c_int32(0)
+    tot = C.c_double(0.0)
+    nbi = nb_idx if len(nb_idx) else np.zeros(1, dtype=np.int32)
+    args = (len(f1_list), ptr(f1_list), ptr(f1_slot), ptr(forests.pix_off), ptr(nb_off), ptr(nbi),
+            int(item_cost), int(tile_cost))
+    check(L.lya_build_work(*args, None, None, C.byref(ni), C.byref(nt), C.byref(tot)))
+    items = np.zeros((max(1, ni.value), 4), dtype=np.int32)
+    tiles = np.zeros((max(1, nt.value), 2), dtype=np.int32)
+    check(L.lya_build_work(*args, ptr(items), ptr(tiles), C.byref(ni), C.byref(nt), C.byref(tot)))
+    return items[:ni.value], tiles[:nt.value], tot.value
+
+
+def default_config(mode="mixed", **kw):
+    cfg = XiConfig()
+    cfg.mode = MODES[mode] if isinstance(mode, str) else int(mode)
+    cfg.px_per_lane = kw.get("px_per_lane", 1)
+    cfg.waves = kw.get("waves", 8)
+    cfg.replicas = kw.get("replicas", 1)
+    cfg.row_pad = kw.get("row_pad", 1)
+    cfg.groups = kw.get("groups", 0)
+    return cfg
+
+
+class XiEngine:
+    """One xi problem resident on the GPU.
+
+    forests   ForestSet (host); uploaded once.
+    angmax    neighbour cut (radians).
+    slots     slot indices whose forests act as forest 1 (default: all).
+    """
+
+    def __init__(self, forests, angmax, np_rp=50, np_rt=50, rpmax=200.0, rtmax=200.0, slots=None,
+                 item_cost=2_000_000, tiles_per_group=6, device="cuda", pairs=None, transposed=False):
+        import torch
+        self.torch = torch
+        self.forests = forests
+        self.angmax = float(angmax)
+        self.np_rp, self.np_rt = int(np_rp), int(np_rt)
+        self.rpmax, self.rtmax = float(rpmax), float(rtmax)
+        self.device = torch.device(device)
+        nb_off, nb_idx = pairs if pairs is not None else build_pairs(forests, angmax)
+        if transposed:
+            nb_off, nb_idx = transpose_pairs(nb_off, nb_idx, forests.nf)
+        self.nb_off, self.nb_idx = nb_off, nb_idx
+        self.n_slots = len(forests.slots)
+        if slots is None:
+            f1_list = np.arange(forests.nf, dtype=np.int32)
+        else:
+            want = np.zeros(self.n_slots, dtype=bool)
+            want[np.asarray(list(slots), dtype=np.int64)] = True
+            f1_list = np.nonzero(want[forests.slot])[0].astype(np.int32)
+        self.f1_list = f1_list
+        self.item_cost = int(item_cost)
+        self.tiles_per_group = tiles_per_group
+        self._work = {}
+        # total pixel pairs over the list (the metric's numerator)
+        _, _, self.pixel_pairs = build_work(forests, nb_off, nb_idx, f1_list, self.item_cost, 1 << 62)
+        self._upload()
+        self._graphs = []
+
+    # ------------------------------------------------------------------
+    def _t(self, a, dtype=None):
+        t = self.torch.from_numpy(np.ascontiguousarray(a))
+        if dtype is not None:
+            t = t.to(dtype)
+        return t.to(self.device)
+
+    def _upload(self):
+        torch = self.torch
+        f = self.forests
+        self.d = {
+            "pix_off": self._t(f.pix_off), "ux": self._t(f.ux), "uy": self._t(f.uy), "uz": self._t(f.uz),
+            "base": self._t(f.base), "sorted": self._t(f.sorted),
+            "dc": self._t(f.dc), "we": self._t(f.we), "dw": self._t(f.dw),
+            "off32": self._t(f.off32), "we32": self._t(f.we32), "dw32": self._t(f.dw32),
+            "nb_idx": self._t(self.nb_idx if len(self.nb_idx) else np.zeros(1, np.int32)),
+        }
+        nbins = self.np_rp * self.np_rt
+        self.out_cnt = torch.zeros((self.n_slots, nbins), dtype=torch.int64, device=self.device)
+        self.out_w = torch.zeros((self.n_slots, nbins), dtype=torch.float64, device=self.device)
+        self.out_dw = torch.zeros((self.n_slots, nbins), dtype=torch.float64, device=self.device)
+        self.queue = torch.zeros(4, dtype=torch.int32, device=self.device)
+        self.tau = self._tau()
+
+    def _tau(self):
+        p = XiProblem()
+        p.np_rp, p.np_rt = self.np_rp, self.np_rt
+        t = C.c_double(0.0)
+        check(lib().lya_xi_tau(C.byref(p), self.forests.max_abs_off, self.np_rp / self.rpmax,
+                               self.np_rt / self.rtmax, C.byref(t)))
+        # round up to the next float32 so the kernel band is never narrower than the bound
+        tf = np.float32(t.value)
+        if float(tf) < t.value:
+            tf = np.nextafter(tf, np.float32(1.0))
+        return float(tf)
+
+    def work(self, cfg):
+        groups = C.c_int32(0)
+        check(lib().lya_xi_grid(C.byref(cfg), C.byref(groups)))
+        key = groups.value
+        if key not in self._work:
+            tile_cost = max(self.item_cost, int(self.pixel_pairs / max(1, key * self.tiles_per_group)))
+            items, tiles, _ = build_work(self.forests, self.nb_off, self.nb_idx, self.f1_list, self.item_cost,
+                                         tile_cost)
+            self._work[key] = (self._t(items if len(items) else np.zeros((1, 4), np.int32)),
+                               self._t(tiles if len(tiles) else np.zeros((1, 2), np.int32)),
+                               len(items), len(tiles))
+        return self._work[key]
+
+    def problem(self, cfg):
+        items, tiles, ni, nt = self.work(cfg)
+        d = self.d
+        p = XiProblem()
+        p.n_forest, p.n_slots = self.forests.nf, self.n_slots
+        for k in ("pix_off", "ux", "uy", "uz", "base", "sorted", "dc", "we", "dw", "off32", "we32", "dw32", "nb_idx"):
+            setattr(p, k, ptr(d[k]))
+        p.items, p.tiles, p.n_items, p.n_tiles = ptr(items), ptr(tiles), ni, nt
+        p.np_rp, p.np_rt, p.rpmax, p.rtmax = self.np_rp, self.np_rt, self.rpmax, self.rtmax
+        p.tau = self.tau
+        p.out_cnt, p.out_w, p.out_dw, p.queue = ptr(self.out_cnt), ptr(self.out_w), ptr(self.out_dw), ptr(self.queue)
+        self._keep = (items, tiles)
+        return p
+
+    def run(self, cfg=None, stream=None):
+        cfg = cfg or default_config()
+        p = self.problem(cfg)
+        check(lib().lya_xi_run(C.byref(p), C.byref(cfg), _lib.stream_ptr(stream)))
+
+    def graph(self, cfg=None):
+        cfg = cfg or default_config()
+        return XiGraph(self, cfg)
+
+    def results(self):
+        """(cnt uint64, w f64, dw f64) host arrays of shape [n_slots, np_rp, np_rt]."""
+        self.torch.cuda.synchronize()
+        sh = (self.n_slots, self.np_rp, self.np_rt)
+        cnt = self.out_cnt.cpu().numpy().view(np.uint64).reshape(sh)
+        return cnt, self.out_w.cpu().numpy().reshape(sh), self.out_dw.cpu().numpy().reshape(sh)
+
+
+class XiGraph:
+    """The zeroing + persistent-kernel sequence captured once as a hipGraph."""
+
+    def __init__(self, eng, cfg):
+        self.eng = eng
+        self.cfg = cfg
+        self.p = eng.problem(cfg)
+        h = C.c_void_p(0)
+        check(lib().lya_xi_graph_create(C.byref(self.p), C.byref(cfg), None, C.byref(h)))
+        self.h = h
+
+    def launch(self, stream=None):
+        check(lib().lya_graph_launch(self.h, _lib.stream_ptr(stream)))
+
+    def close(self):
+        if self.h:
+            lib().lya_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
