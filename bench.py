@@ -1,0 +1,185 @@
+"""Benchmark: pixel-pairs/s of the xi(r_par, r_perp) pair histograms on one MI355X.
+
+Workload (default): 100k synthetic forests over 2,500 deg^2, z_q ~ U[2.1, 3.5],
+HEALPix nside 16, 50x50 bins of 4 Mpc/h, w and delta stored as float32
+(recipe in lya2pcf_amd/synth.py).  A step is one replay of the captured graph
+(zero outputs + persistent kernel) over the whole pair list, with every input
+already in HBM.  value = P / (seconds per step), P = sum of n1*n2 over the
+forest-pair list (pruned pairs included).
+
+The JSON line carries the headline (mixed-precision) kernel and, alongside, the
+plain FP64 kernel on the same data, their bin-count agreement and the
+xi error, the roofline of the mixed kernel (VALU-bound: 10 algorithmic FLOP per
+pixel pair against the FP32 vector peak) and a CPU baseline: the oracle's
+restatement of the reference CPU loop, timed on a bounded sample on the host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "pixel-pairs/s for ξ(r∥,r⊥); mixed vs plain FP64 kernel at ≤1e-6 rel err"
+FLOP_PER_PAIR = 10.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_graph(g, eng, steps, warmup, torch):
+    """Wall time per step over exactly `steps` replays and per-launch event times."""
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        g.launch(stream)
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        starts[k].record(stream)
+        g.launch(stream)
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    ev = [s.elapsed_time(e) / 1e3 for s, e in zip(starts, ends)]
+    return wall, ev
+
+
+def cpu_baseline(fs, eng, angmax, budget_pairs, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+    from parameters import chiquito
+    n = np.diff(fs.pix_off)
+    nb_cnt = np.diff(eng.nb_off)
+    # cost per forest 1 (sum of n1*n2 over its neighbours)
+    f1 = np.repeat(np.arange(fs.nf), nb_cnt)
+    cost = np.bincount(f1, weights=(n[f1] * n[eng.nb_idx]).astype(np.float64), minlength=fs.nf)
+    # a spread-out sample: every k-th forest until the budget is reached
+    order = np.random.default_rng(0).permutation(fs.nf)
+    csum = np.cumsum(cost[order])
+    m = int(np.searchsorted(csum, budget_pairs)) + 1
+    sample = np.sort(order[:m]).astype(np.int32)
+    P = float(cost[sample].sum())
+    t0 = time.perf_counter()
+    oracle.xi_cpu_form(fs, eng.nb_off, eng.nb_idx, angmax, chiquito, f1_list=sample, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": P / dt, "unit": "pixel-pairs/s", "cores": threads, "kind": "port",
+            "sample": "%d of %d forest-1 lines (%.3g pixel pairs, %.1f s), reference CPU formula "
+                      "(correlation_procedures_cpu.py:46-84) restated in C, FP64, OpenMP over forest 1"
+                      % (m, fs.nf, P, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c3"])
+    ap.add_argument("--forests", type=int, default=0, help="override the number of forests (same density)")
+    ap.add_argument("--cpu-pairs", type=float, default=6e10, help="CPU baseline sample size in pixel pairs")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--only", choices=["mixed", "fp64"], default=None, help="time one kernel only (profiling)")
+    ap.add_argument("--ppl", type=int, default=0)
+    ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--replicas", type=int, default=0)
+    ap.add_argument("--row-pad", type=int, default=-1)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from lya2pcf_amd import autotune, forests, synth, xi
+
+    torch.cuda.init()
+    cfgname = args.config
+    cfg = synth.CONFIGS[cfgname]
+    if args.forests:
+        cfg = synth.scaled(cfg, args.forests)
+    t0 = time.perf_counter()
+    survey = synth.generate(cfg)
+    fs = forests.ForestSet.from_survey(survey, float32_inputs=True)
+    del survey
+    angmax = forests.angmax_from(fs, 200.0)
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    eng = xi.XiEngine(fs, angmax)
+    t_setup = time.perf_counter() - t0
+    P = eng.pixel_pairs
+    log("data: %d forests, %d pixels, %d forest pairs, P=%.4g pixel pairs (gen %.1fs, pairs+upload %.1fs)"
+        % (fs.nf, fs.n_pix, len(eng.nb_idx), P, t_gen, t_setup))
+
+    best = autotune.load_best()
+    shape = {k: v for k, v in (best or {}).items() if k in ("px_per_lane", "waves", "replicas", "row_pad")}
+    for k, a in (("px_per_lane", args.ppl), ("waves", args.waves), ("replicas", args.replicas)):
+        if a:
+            shape[k] = a
+    if args.row_pad >= 0:
+        shape["row_pad"] = args.row_pad
+
+    out = {}
+    modes = [args.only] if args.only else (["mixed"] if args.no_fp64 else ["mixed", "fp64"])
+    for mode in modes:
+        g = eng.graph(xi.default_config(mode, **shape))
+        wall, ev = time_graph(g, eng, args.steps, args.warmup, torch)
+        res = eng.results()
+        g.close()
+        out[mode] = {"wall": wall, "ev": ev, "res": res}
+        log("%s: %.4f s/step (events median %.4f s) -> %.4g pixel-pairs/s"
+            % (mode, wall, float(np.median(ev)), P / wall))
+
+    main_mode = "mixed" if "mixed" in out else modes[0]
+    m = out[main_mode]
+    value = P / m["wall"]
+    ev_med = float(np.median(m["ev"]))
+    achieved = P * FLOP_PER_PAIR / ev_med / 1e12
+    peak = FP32_PEAK_TFLOPS if main_mode == "mixed" else FP64_PEAK_TFLOPS
+    line = {
+        "metric": METRIC, "value": value, "unit": "pixel-pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": m["wall"] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32+fp64" if main_mode == "mixed" else "f64", "data": "synthetic",
+        "config": {"workload": cfg.name, "forests": fs.nf, "pixels": fs.n_pix, "forest_pairs": int(len(eng.nb_idx)),
+                   "pixel_pairs": P, "bins": "50x50 of 4 Mpc/h", "kernel": main_mode, "shape": shape,
+                   "parallelism": "single GPU, persistent work queue"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": autotune.measured_traffic(main_mode),
+                     "note": "10 FLOP per pixel pair (algorithmic) / median per-launch hipEvent time; "
+                             "no MFMA or HBM bound applies"},
+    }
+    if "fp64" in out and "mixed" in out:
+        c64, w64, d64 = out["fp64"]["res"]
+        cm, wm, dm = out["mixed"]["res"]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            x64 = np.where(w64 > 0, d64 / w64, 0.0)
+            xm = np.where(wm > 0, dm / wm, 0.0)
+        W64, Wm = w64.sum(0), wm.sum(0)
+        X64 = np.where(W64 > 0, d64.sum(0) / np.where(W64 > 0, W64, 1), 0.0)
+        Xm = np.where(Wm > 0, dm.sum(0) / np.where(Wm > 0, Wm, 1), 0.0)
+        sel = np.abs(X64) > 0
+        line["fp64"] = {"value": P / out["fp64"]["wall"], "ms_per_step": out["fp64"]["wall"] * 1e3,
+                        "speedup_mixed_over_fp64": out["fp64"]["wall"] / out["mixed"]["wall"],
+                        "counts_bit_exact": bool(np.array_equal(c64, cm)),
+                        "xi_total_max_rel_err": float(np.max(np.abs(Xm - X64)[sel] / np.abs(X64[sel]))),
+                        "xi_per_pixel_max_abs_err": float(np.max(np.abs(xm - x64)))}
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log("cpu baseline: %d threads, ~%.2g pixel pairs" % (threads, args.cpu_pairs))
+        line["cpu_baseline"] = cpu_baseline(fs, eng, angmax, args.cpu_pairs, threads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

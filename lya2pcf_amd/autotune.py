@@ -1,0 +1,103 @@
+"""Autotuner for the persistent pair-binning kernel.
+
+Search space (all shapes give identical bins):
+  px_per_lane  1, 2, 4   -> forest-1 tile of 64, 128, 256 pixels per wave
+  replicas     1, 2, 4   -> LDS histogram copies (waves w, w+R, ... share one)
+  row_pad      0, 1      -> LDS row stride np_rt or np_rt + 1 (bank padding)
+  waves        4, 8      -> waves per workgroup
+Each candidate is timed as hipGraph replays of the whole problem; the best
+shape per mode is written to ``tuned.json`` next to this file and used as the
+default by the benchmark.
+
+    python -m lya2pcf_amd.autotune [--forests N] [--modes mixed,fp64]
+"""
+import itertools
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CACHE = os.path.join(HERE, "tuned.json")
+PROFILES = os.path.join(os.path.dirname(HERE), "profiles")
+
+SPACE = {"px_per_lane": (1, 2, 4), "replicas": (1, 2, 4), "row_pad": (0, 1), "waves": (4, 8)}
+
+
+def load_best(mode="mixed"):
+    try:
+        with open(CACHE) as f:
+            return json.load(f).get(mode, {}).get("shape")
+    except (OSError, ValueError):
+        return None
+
+
+def measured_traffic(mode="mixed"):
+    """HBM bytes per launch from the committed rocprofv3 counter summary, if any."""
+    path = os.path.join(PROFILES, "traffic_xi_%s_c1.json" % mode)
+    try:
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def candidates():
+    for ppl, rep, pad, waves in itertools.product(*SPACE.values()):
+        if rep > waves:
+            continue
+        yield {"px_per_lane": ppl, "replicas": rep, "row_pad": pad, "waves": waves}
+
+
+def tune(eng, mode, reps=3, log=print):
+    import torch
+    from . import xi
+    results = []
+    for shape in candidates():
+        try:
+            g = eng.graph(xi.default_config(mode, **shape))
+        except Exception as e:  # shape not supported (LDS size)
+            log("skip %s: %s" % (shape, e))
+            continue
+        g.launch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g.launch()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        g.close()
+        results.append((dt, shape))
+        log("%s %s: %.4f s -> %.4g pixel-pairs/s" % (mode, shape, dt, eng.pixel_pairs / dt))
+    results.sort(key=lambda r: r[0])
+    return results
+
+
+def main():
+    import argparse
+    sys.path.insert(0, os.path.dirname(HERE))
+    from lya2pcf_amd import forests, synth, xi
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default="c1")
+    ap.add_argument("--forests", type=int, default=30000)
+    ap.add_argument("--modes", default="mixed,fp64")
+    ap.add_argument("--reps", type=int, default=3)
+    a = ap.parse_args()
+    cfg = synth.scaled(synth.CONFIGS[a.config], a.forests)
+    fs = forests.ForestSet.from_survey(synth.generate(cfg), float32_inputs=True)
+    eng = xi.XiEngine(fs, forests.angmax_from(fs, 200.0))
+    print("tuning on %d forests, %.3g pixel pairs" % (fs.nf, eng.pixel_pairs), flush=True)
+    out = {}
+    for mode in a.modes.split(","):
+        res = tune(eng, mode, a.reps, log=lambda s: print(s, flush=True))
+        out[mode] = {"shape": res[0][1], "seconds": res[0][0], "pixel_pairs": eng.pixel_pairs,
+                     "workload": cfg.name + " (%d forests)" % fs.nf,
+                     "all": [{"shape": s, "seconds": t} for t, s in res]}
+    path = os.environ.get("LYA_TUNE_OUT", CACHE)
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print("wrote", path)
+
+
+if __name__ == "__main__":
+    main()

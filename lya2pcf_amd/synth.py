@@ -255,3 +255,72 @@ def write_data_npy(survey, path):
     """Write the survey as the reference's pickled ``data#.npy``
     (reference: delta_reader.py:139-146)."""
     np.save(path, to_data_dict(survey), allow_pickle=True)
+
+
+def adversarial(n_pairs, seed=1002, n_pix=8, eps=1e-6, rmax=200.0, nbins=50, omega_m=0.315, nside=16):
+    """Forest pairs whose pixel pairs sit within ``eps`` Mpc/h of bin edges.
+
+    Each pair (f1, f2) gets a random small separation theta; forest 1 has
+    ``n_pix`` pixels spaced ~2 Mpc/h; every forest-2 pixel is placed so that
+    its pair with one chosen forest-1 pixel has r_par (even pixels) or r_perp
+    (odd pixels) equal to an edge 4k + e, e ~ U(-eps, eps) (one pixel in four
+    with e = 0, i.e. on the edge up to FP64 rounding).  The half-angle factors
+    are computed from the stored unit vectors with the FP64 formula the
+    kernels use, so the targets are edges of the evaluated coordinates.
+    Pairs sit at random sky positions, far apart from each other.
+    """
+    rng = np.random.default_rng(seed)
+    bsz = rmax / nbins
+    nf = 2 * n_pairs
+    ra = np.empty(nf); dec = np.empty(nf)
+    dcs = []
+    for k in range(n_pairs):
+        ra1 = rng.uniform(0.2, 2 * np.pi - 0.2)
+        dec1 = np.arcsin(rng.uniform(-0.9, 0.9))
+        th = rng.uniform(5e-4, 0.04)
+        # forest 2 at angle th, displaced mostly in ra so that ra2 > ra1
+        phi = rng.uniform(-0.6, 0.6)
+        dec2 = dec1 + th * np.sin(phi)
+        ra2 = ra1 + th * np.cos(phi) / np.cos(dec1)
+        ra[2 * k], dec[2 * k], ra[2 * k + 1], dec[2 * k + 1] = ra1, dec1, ra2, dec2
+    ux = np.cos(ra) * np.cos(dec); uy = np.sin(ra) * np.cos(dec); uz = np.sin(dec)
+    for k in range(n_pairs):
+        a, b = 2 * k, 2 * k + 1
+        cos12 = (ux[a] * ux[b] + uy[a] * uy[b]) + uz[a] * uz[b]
+        cos12 = min(1.0, max(-1.0, cos12))
+        c, s = np.sqrt(0.5 * (1.0 + cos12)), np.sqrt(0.5 * (1.0 - cos12))
+        r0 = rng.uniform(3700.0, 4800.0)
+        r1 = r0 + np.sort(rng.uniform(0.0, 2.0 * n_pix, n_pix))
+        r2 = np.empty(n_pix)
+        for j in range(n_pix):
+            i = rng.integers(n_pix)
+            e = 0.0 if rng.random() < 0.25 else rng.uniform(-eps, eps)
+            if j % 2 == 0 or s * 2 * r1[i] > rmax - 2 * bsz:
+                kk = rng.integers(1, nbins)
+                d = (kk * bsz + e) / c
+                r2[j] = r1[i] + d if rng.random() < 0.5 else r1[i] - d
+            else:
+                lo = int(np.ceil(s * 2 * r1[i] / bsz)) + 1
+                kk = rng.integers(lo, nbins) if lo < nbins else nbins - 1
+                r2[j] = (kk * bsz + e) / s - r1[i]
+        dcs.append(r1)
+        dcs.append(np.sort(r2))
+    n = np.array([len(x) for x in dcs])
+    pix_off = np.zeros(nf + 1, dtype=np.int64)
+    np.cumsum(n, out=pix_off[1:])
+    dc = np.concatenate(dcs)
+    P = len(dc)
+    we = 1.0 / (0.05 + rng.uniform(0.0, 0.2, P))
+    dw = we * rng.normal(0.0, 0.2, P)
+    hp = healpix.ang2pix_ring(nside, np.pi / 2 - dec, ra)
+    order = np.argsort(hp, kind="stable")
+    sel = np.concatenate([np.arange(pix_off[f], pix_off[f + 1]) for f in order])
+    n2 = n[order]
+    off2 = np.zeros(nf + 1, dtype=np.int64)
+    np.cumsum(n2, out=off2[1:])
+    zero_f = np.zeros(nf)
+    return Survey(ra=ra[order], dec=dec[order], zq=zero_f, name=np.arange(nf, dtype=np.int64)[order] + 7_000_000,
+                  hpix=hp[order], ux=ux[order], uy=uy[order], uz=uz[order], omega=zero_f.copy(),
+                  omega_dl2=zero_f.copy(), lambda_average=zero_f.copy(), pix_off=off2, loglam=np.zeros(P),
+                  z=np.zeros(P), dc=dc[sel], delta=np.zeros(P), sigma2_pip=np.zeros(P), we=we[sel], dw=dw[sel],
+                  delta_lambda=np.zeros(P), nside=nside, config=None)

@@ -1,0 +1,92 @@
+"""Adjoint of the pair sums and the chi^2 gradient.
+
+No reference counterpart exists; pinned by (a) the oracle's FP64 adjoint
+(oracle_xi_adjoint), (b) central finite differences in FP64 (step 1e-3 on the
+weight-model parameters), (c) torch autograd of the per-forest maps."""
+import numpy as np
+import pytest
+
+import oracle
+from lya2pcf_amd import adjoint, forests, synth
+
+
+def test_weight_model_and_projection_cpu():
+    import torch
+    s = synth.generate(synth.scaled(synth.CONFIGS["c0"], 30, seed=4))
+    wm = adjoint.WeightModel(s.z, s.sigma2_pip, s.config.z_nodes, "cpu")
+    w = wm(torch.tensor(1.0, dtype=torch.float64), torch.full((10,), 0.05, dtype=torch.float64))
+    np.testing.assert_allclose(w.numpy(), s.we, rtol=1e-15)
+    pr = adjoint.Projection(s.pix_off, s.loglam, "cpu")
+    dw = pr(torch.from_numpy(s.delta), w)
+    np.testing.assert_allclose(dw.numpy(), s.dw, rtol=1e-9, atol=1e-12)
+    # projection annihilates a + b * loglam
+    z = pr(3.0 + 5.0 * torch.from_numpy(s.loglam), w)
+    assert float(z.abs().max()) < 1e-9
+
+
+def _setup(n=80, seed=17):
+    s = synth.generate(synth.scaled(synth.CONFIGS["c0"], n, seed=seed, length_range=(40, 80)))
+    fs = forests.ForestSet.from_survey(s, float32_inputs=True)
+    return s, fs, forests.angmax_from(fs, 200.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_adjoint_kernel_matches_oracle(mode):
+    import torch
+    s, fs, am = _setup()
+    eng = adjoint.AdjointEngine(fs, am, mode=mode)
+    rng = np.random.default_rng(1)
+    a = rng.normal(size=2500)
+    b = rng.normal(size=2500)
+    g_dw, g_w = eng.adjoint(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda())
+    torch.cuda.synchronize()
+    e_dw, e_w = oracle.xi_adjoint(fs, eng.fwd.nb_off, eng.fwd.nb_idx, a, b)
+    s_dw, s_w = oracle.xi_adjoint(fs, eng.fwd.nb_off, eng.fwd.nb_idx, np.abs(a), np.abs(b),
+                                  we=np.abs(fs.we), dw=np.abs(fs.dw))
+    assert np.all(np.abs(g_dw.cpu().numpy() - e_dw) <= 1e-12 * s_dw + 1e-300)
+    assert np.all(np.abs(g_w.cpu().numpy() - e_w) <= 1e-12 * s_w + 1e-300)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_chi2_gradient_vs_finite_differences(mode):
+    import torch
+    s, fs, am = _setup()
+    dev = "cuda"
+    eng = adjoint.AdjointEngine(fs, am, mode=mode)
+    ref = adjoint.AdjointEngine(fs, am, mode="fp64", pairs=(eng.fwd.nb_off, eng.fwd.nb_idx))
+    wm = adjoint.WeightModel(s.z, s.sigma2_pip, s.config.z_nodes, dev)
+    pr = adjoint.Projection(s.pix_off, s.loglam, dev)
+    m = torch.from_numpy(adjoint.model_xi()).to(dev)
+    delta = torch.from_numpy(s.delta.astype(np.float32).astype(np.float64)).to(dev)
+
+    def chi2(eta, s2, e):
+        w = wm(eta, s2)
+        dw = pr(delta, w)
+        W, DW = adjoint.pair_sums(dw, w, e)
+        return adjoint.chi2(W, DW, m, 1e-4)
+
+    eta = torch.tensor(1.0, dtype=torch.float64, device=dev, requires_grad=True)
+    s2 = torch.full((10,), 0.05, dtype=torch.float64, device=dev, requires_grad=True)
+    L = chi2(eta, s2, eng)
+    L.backward()
+    grads = np.concatenate([[float(eta.grad)], s2.grad.cpu().numpy()])
+    h = 1e-3
+    fd = []
+    with torch.no_grad():
+        for k in range(11):
+            def f(sign):
+                e = torch.tensor(1.0, dtype=torch.float64, device=dev)
+                v = torch.full((10,), 0.05, dtype=torch.float64, device=dev)
+                if k == 0:
+                    e = e + sign * h
+                else:
+                    v[k - 1] += sign * h
+                return float(chi2(e, v, ref))
+            fd.append((f(1) - f(-1)) / (2 * h))
+    fd = np.array(fd)
+    print(mode, "chi2", float(L), "grad", grads, "fd", fd, flush=True)
+    used = np.abs(fd) > 1e-8 * np.abs(fd).max()
+    rel = np.abs(grads - fd)[used] / np.abs(fd[used])
+    assert np.max(rel) < 1e-5, rel

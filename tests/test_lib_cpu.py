@@ -1,0 +1,72 @@
+"""C-ABI library: loads, exports every declared symbol, host-side entry points."""
+import os
+import re
+
+import numpy as np
+
+import oracle
+from lya2pcf_amd import _lib, forests, xi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_exports_match_header():
+    hdr = open(os.path.join(ROOT, "include", "lya2pcf.h")).read()
+    declared = set(re.findall(r"^\s*(?:const char\*|int64_t|int)\s+(lya_\w+)\s*\(", hdr, re.M))
+    assert declared == set(_lib.EXPORTS), declared ^ set(_lib.EXPORTS)
+    L = _lib.lib()
+    for name in declared:
+        assert hasattr(L, name), name
+    assert L.lya_version() >= 1
+
+
+def test_pairs_match_bruteforce(small_set, small_pairs):
+    am, (off, idx) = small_pairs
+    off2, idx2 = oracle.pairs(small_set, am)
+    off3, idx3 = oracle.numpy_pairs(small_set, am)
+    assert np.array_equal(off, off2) and np.array_equal(idx, idx2)
+    assert np.array_equal(off, off3) and np.array_equal(idx, idx3)
+
+
+def test_pair_rule_details():
+    """name equality and ra ties drop a pair (forest_class.py:96)."""
+    fs = forests.ForestSet(np.array([0, 2, 4, 6, 8]), [0], ux=np.array([1.0, 1.0, 1.0, 1.0]),
+                           uy=np.zeros(4), uz=np.zeros(4), ra=np.array([0.1, 0.1, 0.2, 0.3]),
+                           dec=np.zeros(4), name=np.array([5, 6, 7, 7]), slot=np.zeros(4, np.int32),
+                           omega=np.ones(4), odl2=np.ones(4), dc=np.arange(8.0) + 4000, we=np.ones(8),
+                           dw=np.ones(8), dl=np.zeros(8))
+    off, idx = xi.build_pairs(fs, 0.01)
+    lists = [list(idx[off[i]:off[i + 1]]) for i in range(4)]
+    # 0 and 1 tie in ra -> no pair; 2 and 3 share a name -> no pair
+    assert lists == [[2, 3], [2, 3], [], []]
+
+
+def test_build_work_covers_pairs(small_set, small_pairs):
+    am, (off, idx) = small_pairs
+    f1 = np.arange(small_set.nf, dtype=np.int32)
+    items, tiles, tot = xi.build_work(small_set, off, idx, f1, 300_000, 3_000_000)
+    # every neighbour range covered once, items inside one slot per tile
+    cover = np.zeros(len(idx), dtype=np.int64)
+    for f, b, e, s in items:
+        assert off[f] <= b < e <= off[f + 1]
+        assert s == small_set.slot[f]
+        cover[b:e] += 1
+    assert np.all(cover == 1)
+    seen = np.zeros(len(items), dtype=np.int64)
+    for b, e in tiles:
+        assert len(set(items[b:e, 3])) == 1
+        seen[b:e] += 1
+    assert np.all(seen == 1)
+    n = np.diff(small_set.pix_off)
+    f1r = np.repeat(np.arange(small_set.nf), np.diff(off))
+    assert tot == float((n[f1r] * n[idx]).sum())
+
+
+def test_tau_bound_small():
+    """The FP64 recheck band is a few 1e-5 bins for these magnitudes."""
+    import ctypes as C
+    p = _lib.XiProblem()
+    p.np_rp = p.np_rt = 50
+    t = C.c_double()
+    _lib.check(_lib.lib().lya_xi_tau(C.byref(p), 210.0, 0.25, 0.25, C.byref(t)))
+    assert 1e-6 < t.value < 1e-4

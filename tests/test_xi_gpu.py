@@ -1,0 +1,131 @@
+"""GPU parity of the pair-binning kernels against the CPU oracle.
+
+Bar: per-bin pair counts bit-exact in both precision modes; FP64 sums within
+1e-12 of the per-bin absolute sums; mixed sums within 1e-6 of them (FP32
+products and FP32 LDS partials, flushed to FP64 every work item).
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+from lya2pcf_amd import forests, synth, xi
+
+pytestmark = pytest.mark.gpu
+
+
+def _abs_sums(fs, off, idx):
+    """Per-bin sums of |w1 w2| and |dw1 dw2| (error scale of the sums)."""
+    _, aw, ad = oracle.xi_gpu_form(fs, off, idx, we=np.abs(fs.we), dw=np.abs(fs.dw))
+    return aw, ad
+
+
+def _check(fs, off, idx, got, tol, label):
+    c0, w0, d0 = oracle.xi_gpu_form(fs, off, idx)
+    aw, ad = _abs_sums(fs, off, idx)
+    c, w, d = got
+    print(label, "pairs", int(c0.sum()), flush=True)
+    assert np.array_equal(c, c0), "%s: counts differ in %d bins" % (label, int((c != c0).sum()))
+    assert np.all(np.abs(w - w0) <= tol * aw + 1e-300), label
+    assert np.all(np.abs(d - d0) <= tol * ad + 1e-300), label
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_xi_matches_oracle(small_set, small_pairs, mode):
+    am, (off, idx) = small_pairs
+    fs = small_set
+    if mode == "mixed":
+        # identical input values for both: w, dw rounded to float32
+        fs = forests.ForestSet(fs.pix_off, fs.slots, **{k: getattr(fs, k) for k in fs.FOREST + fs.PIXEL})
+        fs.we = fs.we32.astype(np.float64)
+        fs.dw = fs.dw32.astype(np.float64)
+    eng = xi.XiEngine(fs, am, pairs=(off, idx))
+    eng.run(xi.default_config(mode))
+    _check(fs, off, idx, eng.results(), 1e-12 if mode == "fp64" else 1e-6, mode)
+
+
+def test_kernel_shapes(small_set, small_pairs):
+    """Every tile size / workgroup / LDS layout gives the same counts."""
+    am, (off, idx) = small_pairs
+    c0, _, _ = oracle.xi_gpu_form(small_set, off, idx)
+    eng = xi.XiEngine(small_set, am, pairs=(off, idx))
+    for mode, ppl, waves, rep, pad in itertools.product(["fp64", "mixed"], [1, 2, 4], [4, 8], [1, 2], [0, 1]):
+        cfg = xi.default_config(mode, px_per_lane=ppl, waves=waves, replicas=rep, row_pad=pad)
+        eng.run(cfg)
+        c, _, _ = eng.results()
+        assert np.array_equal(c, c0), (mode, ppl, waves, rep, pad)
+
+
+def test_adversarial_edges_bit_exact():
+    """Pixel pairs within 1e-6 Mpc/h of bin edges: mixed bins equal FP64 bins."""
+    s = synth.adversarial(2000, seed=1002)
+    fs = forests.ForestSet.from_survey(s, float32_inputs=True)
+    am = 0.06
+    off, idx = xi.build_pairs(fs, am)
+    c0, _, _ = oracle.xi_gpu_form(fs, off, idx)
+    eng = xi.XiEngine(fs, am, pairs=(off, idx))
+    for mode in ("mixed", "fp64"):
+        for ppl in (1, 2):
+            eng.run(xi.default_config(mode, px_per_lane=ppl))
+            c, _, _ = eng.results()
+            assert np.array_equal(c, c0), (mode, ppl, int((c != c0).sum()))
+
+
+def test_graph_replays(small_set, small_pairs):
+    import torch
+    am, (off, idx) = small_pairs
+    c0, _, _ = oracle.xi_gpu_form(small_set, off, idx)
+    eng = xi.XiEngine(small_set, am, pairs=(off, idx))
+    g = eng.graph(xi.default_config("mixed"))
+    ref = None
+    for rep in range(3):
+        g.launch()
+        torch.cuda.synchronize()
+        c, w, d = eng.results()
+        assert np.array_equal(c, c0), rep
+        if ref is None:
+            ref = (w, d)
+        # eager work between replays
+        eng.run(xi.default_config("fp64"))
+        torch.cuda.synchronize()
+    g.close()
+
+
+def test_halfangles_bit_exact():
+    """Device sqrt(0.5(1 +- cos12)) equals the correctly rounded host value."""
+    import torch
+    from lya2pcf_amd import _lib
+    rng = np.random.default_rng(3)
+    x = np.concatenate([1.0 - rng.uniform(0, 2e-3, 200000), rng.uniform(-1, 1, 50000), [1.0, -1.0, 0.0]])
+    xt = torch.from_numpy(x).cuda()
+    c = torch.empty_like(xt)
+    s = torch.empty_like(xt)
+    _lib.check(_lib.lib().lya_debug_halfangles(_lib.ptr(xt), _lib.ptr(c), _lib.ptr(s), len(x), _lib.stream_ptr()))
+    torch.cuda.synchronize()
+    assert np.array_equal(c.cpu().numpy(), np.sqrt(0.5 * (1.0 + x)))
+    assert np.array_equal(s.cpu().numpy(), np.sqrt(0.5 * (1.0 - x)))
+
+
+def test_slot_subset(small_set, small_pairs):
+    """Computing only some slots gives those slots' histograms unchanged."""
+    am, (off, idx) = small_pairs
+    c0, w0, _ = oracle.xi_gpu_form(small_set, off, idx)
+    keep = [0, len(small_set.slots) - 1]
+    eng = xi.XiEngine(small_set, am, pairs=(off, idx), slots=keep)
+    eng.run(xi.default_config("fp64"))
+    c, w, _ = eng.results()
+    for k in range(len(small_set.slots)):
+        if k in keep:
+            assert np.array_equal(c[k], c0[k])
+        else:
+            assert c[k].sum() == 0
+
+
+def test_empty_problem(small_set):
+    """No pairs (angmax tiny): zero histograms, no launch error."""
+    eng = xi.XiEngine(small_set, 1e-9)
+    assert len(eng.nb_idx) == 0
+    eng.run(xi.default_config("mixed"))
+    c, w, d = eng.results()
+    assert c.sum() == 0 and not w.any() and not d.any()
