@@ -88,7 +88,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--only", choices=["mixed", "fp64"], default=None, help="time one kernel only (profiling)")
+    ap.add_argument("--only", choices=["mixed", "fp64", "mixed32"], default=None,
+                    help="time one kernel only (profiling)")
     ap.add_argument("--ppl", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--replicas", type=int, default=0)
@@ -131,7 +132,7 @@ def main():
         shape["row_pad"] = args.row_pad
 
     out = {}
-    modes = [args.only] if args.only else (["mixed"] if args.no_fp64 else ["mixed", "fp64"])
+    modes = [args.only] if args.only else (["mixed"] if args.no_fp64 else ["mixed", "fp64", "mixed32"])
     for mode in modes:
         g = eng.graph(xi.default_config(mode, **shape))
         wall, ev = time_graph(g, eng, args.steps, args.warmup, torch)
@@ -146,11 +147,11 @@ def main():
     value = P / m["wall"]
     ev_med = float(np.median(m["ev"]))
     achieved = P * FLOP_PER_PAIR / ev_med / 1e12
-    peak = FP32_PEAK_TFLOPS if main_mode == "mixed" else FP64_PEAK_TFLOPS
+    peak = FP64_PEAK_TFLOPS if main_mode == "fp64" else FP32_PEAK_TFLOPS
     line = {
         "metric": METRIC, "value": value, "unit": "pixel-pairs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": m["wall"] * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32+fp64" if main_mode == "mixed" else "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f64" if main_mode == "fp64" else "f32+f64", "data": "synthetic",
         "config": {"workload": cfg.name, "forests": fs.nf, "pixels": fs.n_pix, "forest_pairs": int(len(eng.nb_idx)),
                    "pixel_pairs": P, "bins": "50x50 of 4 Mpc/h", "kernel": main_mode, "shape": shape,
                    "parallelism": "single GPU, persistent work queue"},
@@ -174,6 +175,16 @@ def main():
                         "counts_bit_exact": bool(np.array_equal(c64, cm)),
                         "xi_total_max_rel_err": float(np.max(np.abs(Xm - X64)[sel] / np.abs(X64[sel]))),
                         "xi_per_pixel_max_abs_err": float(np.max(np.abs(xm - x64)))}
+    if "mixed32" in out and "fp64" in out:
+        c64, w64, d64 = out["fp64"]["res"]
+        c3, w3, d3 = out["mixed32"]["res"]
+        W64, W3 = w64.sum(0), w3.sum(0)
+        X64 = np.where(W64 > 0, d64.sum(0) / np.where(W64 > 0, W64, 1), 0.0)
+        X3 = np.where(W3 > 0, d3.sum(0) / np.where(W3 > 0, W3, 1), 0.0)
+        sel = np.abs(X64) > 0
+        line["mixed32"] = {"value": P / out["mixed32"]["wall"], "ms_per_step": out["mixed32"]["wall"] * 1e3,
+                           "counts_bit_exact": bool(np.array_equal(c64, c3)),
+                           "xi_total_max_rel_err": float(np.max(np.abs(X3 - X64)[sel] / np.abs(X64[sel])))}
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log("cpu baseline: %d threads, ~%.2g pixel pairs" % (threads, args.cpu_pairs))

@@ -37,7 +37,8 @@ extern "C" {
 #define LYA_ERR_UNSUPPORTED (-4)
 
 #define LYA_MODE_FP64 0   /* plain FP64 pair loop, FP64 LDS histograms */
-#define LYA_MODE_MIXED 1  /* FP32 geometry + FP64 near-edge recheck, FP32 LDS -> FP64 merge */
+#define LYA_MODE_MIXED 1  /* FP32 geometry + FP64 near-edge recheck, exact FP64 products, FP64 LDS sums */
+#define LYA_MODE_MIXED_F32 2  /* FP32 geometry + recheck, FP32 products and LDS partials -> FP64 merge */
 
 /* Largest (numpix_rp * row stride) the LDS histogram holds. */
 #define LYA_MAX_BINS 2560
@@ -108,7 +109,7 @@ typedef struct lya_xi_problem {
 
 /* Kernel shape, chosen by the autotuner.  Zero fields take defaults. */
 typedef struct lya_xi_config {
-    int32_t mode;             /* LYA_MODE_FP64 or LYA_MODE_MIXED                      */
+    int32_t mode;             /* LYA_MODE_FP64, LYA_MODE_MIXED or LYA_MODE_MIXED_F32  */
     int32_t px_per_lane;      /* 1, 2, 4: forest-1 tile of 64, 128, 256 pixels        */
     int32_t waves;            /* waves per workgroup: 4 or 8                          */
     int32_t replicas;         /* LDS histogram copies: 1, 2, 4                        */

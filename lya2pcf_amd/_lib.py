@@ -11,6 +11,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build", "l
 
 LYA_MODE_FP64 = 0
 LYA_MODE_MIXED = 1
+LYA_MODE_MIXED_F32 = 2
 LYA_MAX_BINS = 2560
 
 _vp = C.c_void_p

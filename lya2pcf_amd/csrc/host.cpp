@@ -50,9 +50,9 @@ extern "C" int lya_build_pairs(int32_t n, const double* ux, const double* uy, co
     double cm = std::max(-1.0, std::min(1.0, cos_max));
     double chord = std::sqrt(std::max(0.0, 2.0 - 2.0 * cm));
     Grid g;
-    g.h = std::max(chord * 1.0001 + 1e-9, 1e-6);
+    // cells no smaller than 2/4096: a coarser grid only adds candidates
+    g.h = std::max(chord * 1.0001 + 1e-9, 2.0 / 4096.0);
     g.m = std::max<int64_t>(1, (int64_t)std::ceil(2.0 / g.h) + 1);
-    if (g.m > 2000000) return lya_fail(LYA_ERR_ARG, "lya_build_pairs: cut angle too small for the grid");
 
     std::vector<int64_t> keys(n);
     for (int32_t i = 0; i < n; ++i) keys[i] = g.key(g.coord(ux[i]), g.coord(uy[i]), g.coord(uz[i]));

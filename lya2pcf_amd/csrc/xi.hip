@@ -24,11 +24,14 @@
 //   LYA_MODE_FP64   the reference formula in FP64 (no FMA contraction),
 //                   FP64 LDS sums.
 //   LYA_MODE_MIXED  FP32 geometry on per-forest offsets (dc - base), binner
-//                   folded into the half-angles, FP32 products and FP32 LDS
-//                   partials flushed every item into FP64 LDS sums; any pixel
-//                   pair whose scaled rp or rt lies within the rounding bound
-//                   tau of a bin edge is recomputed with the FP64 formula, so
-//                   the bin indices equal the FP64 ones exactly.
+//                   folded into the half-angles; any pixel pair whose scaled
+//                   rp or rt lies within the rounding bound tau of a bin edge
+//                   is recomputed with the FP64 formula, so the bin indices
+//                   equal the FP64 ones exactly.  Inputs w, dw are FP32; their
+//                   products are formed exactly in FP64 and summed in FP64 LDS.
+//   LYA_MODE_MIXED_F32  the same geometry with FP32 products and FP32 LDS
+//                   partials, flushed every work item into FP64 LDS sums
+//                   (fewer LDS bytes, ~1e-6 relative error on the sums).
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
@@ -79,17 +82,16 @@ __device__ __forceinline__ void lds_add(unsigned char* lds, uint32_t byte_off, T
     __hip_atomic_fetch_add((T*)(lds + byte_off), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// LDS layout (bytes).  MIXED: per replica [cnt u32 | w f32 | dw f32] x HS, then
-// the FP64 accumulators [w f64 | dw f64] x HS.  FP64: per replica
-// [cnt u32 | w f64 | dw f64] x HS.
+// LDS layout (bytes).  MIXED_F32: per replica [cnt u32 | w f32 | dw f32] x HS,
+// then the FP64 accumulators [w f64 | dw f64] x HS.  FP64 and MIXED: per
+// replica [cnt u32 | w f64 | dw f64] x HS.
 template <int MODE>
 struct Layout {
-    static constexpr uint32_t REPL = (MODE == LYA_MODE_MIXED) ? 12u * HS : 20u * HS;
-    static constexpr uint32_t W = 4u * HS;                                      // w offset in a replica
-    static constexpr uint32_t DW = (MODE == LYA_MODE_MIXED) ? 8u * HS : 12u * HS;  // dw offset
-    static __host__ __device__ uint32_t bytes(int R) {
-        return REPL * (uint32_t)R + ((MODE == LYA_MODE_MIXED) ? 16u * HS : 0u);
-    }
+    static constexpr bool F32 = (MODE == LYA_MODE_MIXED_F32);
+    static constexpr uint32_t REPL = F32 ? 12u * HS : 20u * HS;
+    static constexpr uint32_t W = 4u * HS;                   // w offset in a replica
+    static constexpr uint32_t DW = F32 ? 8u * HS : 12u * HS;  // dw offset
+    static __host__ __device__ uint32_t bytes(int R) { return REPL * (uint32_t)R + (F32 ? 16u * HS : 0u); }
     static __host__ __device__ uint32_t acc(int R) { return REPL * (uint32_t)R; }
 };
 
@@ -272,14 +274,29 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
             }
 #pragma unroll
             for (int u = 0; u < UU; ++u) {
+                if constexpr (L::F32) {
 #pragma unroll
-                for (int k = 0; k < PPL; ++k) {
-                    const float pw = w1[k] * w2v[u], pd = d1[k] * d2v[u];
-                    if (bp[u][k] < (uint32_t)NP && bt[u][k] < (uint32_t)NT) {
-                        const uint32_t a = __umul24(bp[u][k], ntpad4) + (bt[u][k] << 2) + hbase;
-                        lds_add<uint32_t>(lds, a, 1u);
-                        lds_add<float>(lds, a + L::W, pw);
-                        lds_add<float>(lds, a + L::DW, pd);
+                    for (int k = 0; k < PPL; ++k) {
+                        const float pw = w1[k] * w2v[u], pd = d1[k] * d2v[u];
+                        if (bp[u][k] < (uint32_t)NP && bt[u][k] < (uint32_t)NT) {
+                            const uint32_t a = __umul24(bp[u][k], ntpad4) + (bt[u][k] << 2) + hbase;
+                            lds_add<uint32_t>(lds, a, 1u);
+                            lds_add<float>(lds, a + L::W, pw);
+                            lds_add<float>(lds, a + L::DW, pd);
+                        }
+                    }
+                } else {
+                    // products of FP32 inputs are exact in FP64
+                    const double w2d = (double)w2v[u], d2d = (double)d2v[u];
+#pragma unroll
+                    for (int k = 0; k < PPL; ++k) {
+                        const double pw = (double)w1[k] * w2d, pd = (double)d1[k] * d2d;
+                        if (bp[u][k] < (uint32_t)NP && bt[u][k] < (uint32_t)NT) {
+                            const uint32_t a4 = __umul24(bp[u][k], ntpad4) + (bt[u][k] << 2);
+                            lds_add<uint32_t>(lds, hbase + a4, 1u);
+                            lds_add<double>(lds, hbase + L::W + 2u * a4, pw);
+                            lds_add<double>(lds, hbase + L::DW + 2u * a4, pd);
+                        }
                     }
                 }
             }
@@ -292,7 +309,7 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
 
 // Mixed mode: move the FP32 partial sums into the FP64 accumulators.
 __device__ void flush_item(unsigned char* lds, int R, int nb_lds, int tid, int nthr) {
-    using L = Layout<LYA_MODE_MIXED>;
+    using L = Layout<LYA_MODE_MIXED_F32>;
     double* aw = (double*)(lds + L::acc(R));
     double* ad = aw + HS;
     for (int b = tid; b < nb_lds; b += nthr) {
@@ -324,7 +341,7 @@ __device__ void flush_tile(const lya_xi_problem& p, unsigned char* lds, int R, i
             uint32_t* hc = (uint32_t*)(lds + r * L::REPL);
             cnt += hc[b];
             hc[b] = 0u;
-            if constexpr (MODE == LYA_MODE_FP64) {
+            if constexpr (!L::F32) {
                 double* hw = (double*)(lds + r * L::REPL + L::W);
                 double* hd = (double*)(lds + r * L::REPL + L::DW);
                 sw += hw[b];
@@ -333,7 +350,7 @@ __device__ void flush_tile(const lya_xi_problem& p, unsigned char* lds, int R, i
                 hd[b] = 0.0;
             }
         }
-        if constexpr (MODE == LYA_MODE_MIXED) {
+        if constexpr (L::F32) {
             double* aw = (double*)(lds + L::acc(R));
             double* ad = aw + HS;
             sw = aw[b];
@@ -395,7 +412,7 @@ __global__ void __launch_bounds__(WAVES * 64) xi_kernel(const lya_xi_problem p, 
                 const int f2 = cptr(p.nb_idx)[nb0 + q];
                 run_task<MODE, PPL>(p, lds, hbase, ntpad4, t, slab, f2, lane, perm, bp64, bt64);
             }
-            if constexpr (MODE == LYA_MODE_MIXED) {
+            if constexpr (MODE == LYA_MODE_MIXED_F32) {
                 __syncthreads();
                 flush_item(lds, R, nb_lds, tid, NTHR);
             }
@@ -544,6 +561,7 @@ XiKernel pick_w(int waves) {
 }
 template <int MODE>
 XiKernel pick_p(int ppl, int waves) {
+    static_assert(MODE == LYA_MODE_FP64 || MODE == LYA_MODE_MIXED || MODE == LYA_MODE_MIXED_F32, "mode");
     if (ppl == 1) return pick_w<MODE, 1>(waves);
     if (ppl == 2) return pick_w<MODE, 2>(waves);
     if (ppl == 4) return pick_w<MODE, 4>(waves);
@@ -562,10 +580,13 @@ int resolve(const lya_xi_problem* p, const lya_xi_config* cfg, Resolved& r) {
     r.waves = (cfg && cfg->waves) ? cfg->waves : 8;
     r.R = (cfg && cfg->replicas) ? cfg->replicas : 1;
     const int pad = (cfg && cfg->row_pad > 0) ? cfg->row_pad : 0;
-    if (r.mode != LYA_MODE_FP64 && r.mode != LYA_MODE_MIXED) return lya_fail(LYA_ERR_ARG, "unknown mode %d", r.mode);
+    if (r.mode != LYA_MODE_FP64 && r.mode != LYA_MODE_MIXED && r.mode != LYA_MODE_MIXED_F32)
+        return lya_fail(LYA_ERR_ARG, "unknown mode %d", r.mode);
     if (r.R != 1 && r.R != 2 && r.R != 4) return lya_fail(LYA_ERR_ARG, "replicas must be 1, 2 or 4");
     if (r.R > r.waves) return lya_fail(LYA_ERR_ARG, "replicas > waves");
-    r.fn = (r.mode == LYA_MODE_FP64) ? pick_p<LYA_MODE_FP64>(r.ppl, r.waves) : pick_p<LYA_MODE_MIXED>(r.ppl, r.waves);
+    r.fn = (r.mode == LYA_MODE_FP64)    ? pick_p<LYA_MODE_FP64>(r.ppl, r.waves)
+           : (r.mode == LYA_MODE_MIXED) ? pick_p<LYA_MODE_MIXED>(r.ppl, r.waves)
+                                        : pick_p<LYA_MODE_MIXED_F32>(r.ppl, r.waves);
     if (!r.fn) return lya_fail(LYA_ERR_ARG, "unsupported shape px_per_lane=%d waves=%d", r.ppl, r.waves);
     if (p) {
         if (p->np_rp < 1 || p->np_rt < 1 || !(p->rpmax > 0) || !(p->rtmax > 0))
@@ -577,7 +598,7 @@ int resolve(const lya_xi_problem* p, const lya_xi_config* cfg, Resolved& r) {
     } else {
         r.ntpad = 0;
     }
-    r.lds = (r.mode == LYA_MODE_FP64) ? Layout<LYA_MODE_FP64>::bytes(r.R) : Layout<LYA_MODE_MIXED>::bytes(r.R);
+    r.lds = (r.mode == LYA_MODE_MIXED_F32) ? Layout<LYA_MODE_MIXED_F32>::bytes(r.R) : Layout<LYA_MODE_FP64>::bytes(r.R);
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
@@ -598,9 +619,9 @@ int check_problem(const lya_xi_problem* p, int mode) {
         !p->tiles || !p->queue)
         return lya_fail(LYA_ERR_ARG, "null device pointer in problem");
     if (mode == LYA_MODE_FP64 && (!p->we || !p->dw)) return lya_fail(LYA_ERR_ARG, "FP64 mode needs we, dw");
-    if (mode == LYA_MODE_MIXED && (!p->off32 || !p->we32 || !p->dw32 || !p->base))
+    if (mode != LYA_MODE_FP64 && (!p->off32 || !p->we32 || !p->dw32 || !p->base))
         return lya_fail(LYA_ERR_ARG, "mixed mode needs off32, we32, dw32, base");
-    if (mode == LYA_MODE_MIXED && !(p->tau > 0.0f && p->tau < 0.25f))
+    if (mode != LYA_MODE_FP64 && !(p->tau > 0.0f && p->tau < 0.25f))
         return lya_fail(LYA_ERR_ARG, "mixed mode needs 0 < tau < 0.25 (got %g)", (double)p->tau);
     if (p->n_tiles < 0 || p->n_items < 0 || p->n_slots < 0) return lya_fail(LYA_ERR_ARG, "negative sizes");
     return LYA_OK;

@@ -14,9 +14,9 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import XiProblem, XiConfig, LYA_MODE_FP64, LYA_MODE_MIXED, check, lib, ptr
+from ._lib import XiProblem, XiConfig, LYA_MODE_FP64, LYA_MODE_MIXED, LYA_MODE_MIXED_F32, check, lib, ptr
 
-MODES = {"fp64": LYA_MODE_FP64, "mixed": LYA_MODE_MIXED}
+MODES = {"fp64": LYA_MODE_FP64, "mixed": LYA_MODE_MIXED, "mixed32": LYA_MODE_MIXED_F32}
 
 
 def build_pairs(forests, angmax):

@@ -72,21 +72,62 @@ def test_chi2_gradient_vs_finite_differences(mode):
     L = chi2(eta, s2, eng)
     L.backward()
     grads = np.concatenate([[float(eta.grad)], s2.grad.cpu().numpy()])
+    # central differences with step h = 1e-3, Richardson-extrapolated with h/2:
+    # the plain step-1e-3 difference carries an O(h^2) truncation error of order
+    # (h / (sigma2_pip + sigma2_lss))^2 ~ 1e-3 here, the extrapolated one O(h^4).
     h = 1e-3
     fd = []
     with torch.no_grad():
         for k in range(11):
-            def f(sign):
+            def f(step):
                 e = torch.tensor(1.0, dtype=torch.float64, device=dev)
                 v = torch.full((10,), 0.05, dtype=torch.float64, device=dev)
                 if k == 0:
-                    e = e + sign * h
+                    e = e + step
                 else:
-                    v[k - 1] += sign * h
+                    v[k - 1] += step
                 return float(chi2(e, v, ref))
-            fd.append((f(1) - f(-1)) / (2 * h))
+            d1 = (f(h) - f(-h)) / (2 * h)
+            d2 = (f(h / 2) - f(-h / 2)) / h
+            fd.append((4 * d2 - d1) / 3)
     fd = np.array(fd)
-    print(mode, "chi2", float(L), "grad", grads, "fd", fd, flush=True)
+    print(mode, "chi2", float(L.detach()), "grad", grads, "fd", fd, flush=True)
     used = np.abs(fd) > 1e-8 * np.abs(fd).max()
     rel = np.abs(grads - fd)[used] / np.abs(fd[used])
     assert np.max(rel) < 1e-5, rel
+
+
+@pytest.mark.gpu
+def test_gradient_wrt_delta_and_w_pixels():
+    """dchi2/ddelta_k and dchi2/dw_k at sampled pixels against central differences."""
+    import torch
+    s, fs, am = _setup(60, 23)
+    dev = "cuda"
+    eng = adjoint.AdjointEngine(fs, am, mode="fp64")
+    pr = adjoint.Projection(s.pix_off, s.loglam, dev)
+    m = torch.from_numpy(adjoint.model_xi()).to(dev)
+    delta0 = torch.from_numpy(s.delta).to(dev)
+    w0 = torch.from_numpy(s.we).to(dev)
+
+    def chi2(delta, w):
+        dw = pr(delta, w)
+        W, DW = adjoint.pair_sums(dw, w, eng)
+        return adjoint.chi2(W, DW, m, 1e-4)
+
+    d = delta0.clone().requires_grad_(True)
+    w = w0.clone().requires_grad_(True)
+    chi2(d, w).backward()
+    rng = np.random.default_rng(5)
+    ks = rng.choice(fs.n_pix, 12, replace=False)
+    with torch.no_grad():
+        for k in ks:
+            for which, base, grad, h in (("delta", delta0, d.grad, 1e-3), ("w", w0, w.grad, 1e-3)):
+                def f(step):
+                    x = base.clone()
+                    x[k] += step
+                    return float(chi2(x, w0) if which == "delta" else chi2(delta0, x))
+                d1 = (f(h) - f(-h)) / (2 * h)
+                d2 = (f(h / 2) - f(-h / 2)) / h
+                fdv = (4 * d2 - d1) / 3
+                g = float(grad[k])
+                assert abs(g - fdv) <= 1e-5 * abs(fdv) + 1e-9 * float(chi2(delta0, w0)), (which, k, g, fdv)
