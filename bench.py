@@ -84,7 +84,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c1", choices=["c0", "c1", "c3"])
     ap.add_argument("--forests", type=int, default=0, help="override the number of forests (same density)")
-    ap.add_argument("--cpu-pairs", type=float, default=6e10, help="CPU baseline sample size in pixel pairs")
+    ap.add_argument("--cpu-pairs", type=float, default=1.5e11, help="CPU baseline sample size in pixel pairs")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -94,6 +94,9 @@ def main():
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--replicas", type=int, default=0)
     ap.add_argument("--row-pad", type=int, default=-1)
+    ap.add_argument("--copy-calibrate", type=int, default=0,
+                    help="after timing, run one plain device copy of this many bytes (calibrates the "
+                         "profiler's HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -123,24 +126,37 @@ def main():
     log("data: %d forests, %d pixels, %d forest pairs, P=%.4g pixel pairs (gen %.1fs, pairs+upload %.1fs)"
         % (fs.nf, fs.n_pix, len(eng.nb_idx), P, t_gen, t_setup))
 
-    best = autotune.load_best()
-    shape = {k: v for k, v in (best or {}).items() if k in ("px_per_lane", "waves", "replicas", "row_pad")}
-    for k, a in (("px_per_lane", args.ppl), ("waves", args.waves), ("replicas", args.replicas)):
-        if a:
-            shape[k] = a
-    if args.row_pad >= 0:
-        shape["row_pad"] = args.row_pad
+    def shape_for(mode):
+        best = autotune.load_best(mode)
+        shape = {k: v for k, v in (best or {}).items() if k in ("px_per_lane", "waves", "replicas", "row_pad")}
+        for k, a in (("px_per_lane", args.ppl), ("waves", args.waves), ("replicas", args.replicas)):
+            if a:
+                shape[k] = a
+        if args.row_pad >= 0:
+            shape["row_pad"] = args.row_pad
+        return shape
 
     out = {}
-    modes = [args.only] if args.only else (["mixed"] if args.no_fp64 else ["mixed", "fp64", "mixed32"])
+    modes = [args.only] if args.only else (["mixed"] if args.no_fp64 else ["mixed", "fp64"])
     for mode in modes:
+        shape = shape_for(mode)
         g = eng.graph(xi.default_config(mode, **shape))
         wall, ev = time_graph(g, eng, args.steps, args.warmup, torch)
         res = eng.results()
         g.close()
-        out[mode] = {"wall": wall, "ev": ev, "res": res}
+        out[mode] = {"wall": wall, "ev": ev, "res": res, "shape": shape}
         log("%s: %.4f s/step (events median %.4f s) -> %.4g pixel-pairs/s"
             % (mode, wall, float(np.median(ev)), P / wall))
+
+    if args.copy_calibrate:
+        from lya2pcf_amd import _lib
+        nbytes = (args.copy_calibrate // 16) * 16
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        _lib.check(_lib.lib().lya_copy_bytes(_lib.ptr(src), _lib.ptr(dst), nbytes, _lib.stream_ptr()))
+        torch.cuda.synchronize()
+        log("copy calibration: %d bytes" % nbytes)
 
     main_mode = "mixed" if "mixed" in out else modes[0]
     m = out[main_mode]
@@ -153,7 +169,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": m["wall"] * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64" if main_mode == "fp64" else "f32+f64", "data": "synthetic",
         "config": {"workload": cfg.name, "forests": fs.nf, "pixels": fs.n_pix, "forest_pairs": int(len(eng.nb_idx)),
-                   "pixel_pairs": P, "bins": "50x50 of 4 Mpc/h", "kernel": main_mode, "shape": shape,
+                   "pixel_pairs": P, "bins": "50x50 of 4 Mpc/h", "kernel": main_mode, "shape": out[main_mode]["shape"],
                    "parallelism": "single GPU, persistent work queue"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": autotune.measured_traffic(main_mode),
@@ -171,6 +187,7 @@ def main():
         Xm = np.where(Wm > 0, dm.sum(0) / np.where(Wm > 0, Wm, 1), 0.0)
         sel = np.abs(X64) > 0
         line["fp64"] = {"value": P / out["fp64"]["wall"], "ms_per_step": out["fp64"]["wall"] * 1e3,
+                        "shape": out["fp64"]["shape"],
                         "speedup_mixed_over_fp64": out["fp64"]["wall"] / out["mixed"]["wall"],
                         "counts_bit_exact": bool(np.array_equal(c64, cm)),
                         "xi_total_max_rel_err": float(np.max(np.abs(Xm - X64)[sel] / np.abs(X64[sel]))),

@@ -37,7 +37,7 @@ extern "C" {
 #define LYA_ERR_UNSUPPORTED (-4)
 
 #define LYA_MODE_FP64 0   /* plain FP64 pair loop, FP64 LDS histograms */
-#define LYA_MODE_MIXED 1  /* FP32 geometry + FP64 near-edge recheck, exact FP64 products, FP64 LDS sums */
+#define LYA_MODE_MIXED 1  /* FP32 geometry + FP64 near-edge recheck, FP64 products and LDS sums */
 #define LYA_MODE_MIXED_F32 2  /* FP32 geometry + recheck, FP32 products and LDS partials -> FP64 merge */
 
 /* Largest (numpix_rp * row stride) the LDS histogram holds. */
@@ -85,11 +85,11 @@ typedef struct lya_xi_problem {
     const double* base;       /* [n_forest] integer-valued origin of off32 (mixed)    */
     const uint8_t* sorted;    /* [n_forest] 1 if dc increases along the forest        */
     const double* dc;         /* [n_pix] comoving distance, FP64                      */
-    const double* we;         /* [n_pix] weight, FP64 (FP64 mode)                     */
-    const double* dw;         /* [n_pix] weight*delta, FP64 (FP64 mode)               */
+    const double* we;         /* [n_pix] weight, FP64 (FP64 and mixed modes)          */
+    const double* dw;         /* [n_pix] weight*delta, FP64 (FP64 and mixed modes)    */
     const float* off32;       /* [n_pix] dc - base[forest], FP32 (mixed)              */
-    const float* we32;        /* [n_pix] weight, FP32 (mixed)                         */
-    const float* dw32;        /* [n_pix] weight*delta, FP32 (mixed)                   */
+    const float* we32;        /* [n_pix] weight, FP32 (mixed32 mode)                  */
+    const float* dw32;        /* [n_pix] weight*delta, FP32 (mixed32 mode)            */
     const int32_t* nb_idx;    /* [n_pairs] neighbour forest indices                   */
     const int32_t* items;     /* [n_items][4] f1, nb_begin, nb_end, slot              */
     const int32_t* tiles;     /* [n_tiles][2] item_begin, item_end                    */
@@ -115,6 +115,7 @@ typedef struct lya_xi_config {
     int32_t replicas;         /* LDS histogram copies: 1, 2, 4                        */
     int32_t row_pad;          /* extra LDS columns per r_parallel row: 0 or 1         */
     int32_t groups;           /* workgroups in the persistent grid (0 = fill the GPU)  */
+    int32_t no_counts;        /* 1: skip the per-bin pair counts (out_cnt left zero)  */
 } lya_xi_config;
 
 /* Largest mixed-mode error bound for these data (host): returns tau in bin units. */

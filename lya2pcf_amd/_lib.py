@@ -36,7 +36,7 @@ class XiProblem(C.Structure):
 class XiConfig(C.Structure):
     """Mirror of ``lya_xi_config``."""
     _fields_ = [("mode", _i32), ("px_per_lane", _i32), ("waves", _i32), ("replicas", _i32),
-                ("row_pad", _i32), ("groups", _i32)]
+                ("row_pad", _i32), ("groups", _i32), ("no_counts", _i32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -2,7 +2,7 @@
 
 Search space (all shapes give identical bins):
   px_per_lane  1, 2, 4   -> forest-1 tile of 64, 128, 256 pixels per wave
-  replicas     1, 2, 4   -> LDS histogram copies (waves w, w+R, ... share one)
+  replicas     1, 2      -> LDS histogram copies (waves w, w+R, ... share one)
   row_pad      0, 1      -> LDS row stride np_rt or np_rt + 1 (bank padding)
   waves        4, 8      -> waves per workgroup
 Each candidate is timed as hipGraph replays of the whole problem; the best
@@ -21,7 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuned.json")
 PROFILES = os.path.join(os.path.dirname(HERE), "profiles")
 
-SPACE = {"px_per_lane": (1, 2, 4), "replicas": (1, 2, 4), "row_pad": (0, 1), "waves": (4, 8)}
+SPACE = {"px_per_lane": (1, 2, 4), "replicas": (1, 2), "row_pad": (0, 1), "waves": (4, 8)}
 
 
 def load_best(mode="mixed"):

@@ -77,6 +77,13 @@ __device__ __forceinline__ void halfangles(double x1, double y1, double z1, doub
     s = sqrt(0.5 * (1.0 - cos12));
 }
 
+// Keeps x as one VGPR value so that a + x*k folds into a single v_mad_u32_u24
+// instead of being reassociated into mul + shift + add3 (no instruction emitted).
+__device__ __forceinline__ uint32_t opaque(uint32_t x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 template <class T>
 __device__ __forceinline__ void lds_add(unsigned char* lds, uint32_t byte_off, T v) {
     __hip_atomic_fetch_add((T*)(lds + byte_off), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -91,8 +98,8 @@ struct Layout {
     static constexpr uint32_t REPL = F32 ? 12u * HS : 20u * HS;
     static constexpr uint32_t W = 4u * HS;                   // w offset in a replica
     static constexpr uint32_t DW = F32 ? 8u * HS : 12u * HS;  // dw offset
-    static __host__ __device__ uint32_t bytes(int R) { return REPL * (uint32_t)R + (F32 ? 16u * HS : 0u); }
-    static __host__ __device__ uint32_t acc(int R) { return REPL * (uint32_t)R; }
+    static constexpr __host__ __device__ uint32_t bytes(int R) { return REPL * (uint32_t)R + (F32 ? 16u * HS : 0u); }
+    static constexpr __host__ __device__ uint32_t acc(int R) { return REPL * (uint32_t)R; }
 };
 
 struct TaskCtx {
@@ -130,8 +137,12 @@ __device__ __forceinline__ void prune(const lya_xi_problem& p, int f1, int64_t p
 
 // ---------------------------------------------------------------------------
 // One task: a 64*PPL-pixel slab of forest 1 against all pixels of forest 2.
+//   CNT   also count pixel pairs per bin (one more LDS atomic per pair)
+//   POW2  FP64 mode with power-of-two binners: the binner is folded into the
+//         half-angle factors, fl(|d| (c b)) = fl(|d| c) b exactly when b = 2^k,
+//         so the bin equals the reference's floor(fl(fl(|d| c) b)).
 // ---------------------------------------------------------------------------
-template <int MODE, int PPL>
+template <int MODE, int PPL, bool CNT, bool POW2>
 __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char* lds, uint32_t hbase,
                                          uint32_t ntpad4, const TaskCtx& t, int slab, int f2, int lane, int perm,
                                          double bp64, double bt64) {
@@ -160,8 +171,14 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
         pix1[k] = t.p1 + i;
         r1[k] = valid[k] ? p.dc[pix1[k]] : 1.0e9;
     }
+    // LDS addresses: count at a4 = hbase + 4 (bp ntpad + bt), sums at
+    // a8 = hbase + W + 8 (bp ntpad + bt) = 2 a4 + (W - hbase).
+    const uint32_t woff = L::W - hbase;
+    const uint32_t ntpad8 = 2u * ntpad4;
+    const uint32_t hbw = hbase + L::W;
 
-    if constexpr (MODE == LYA_MODE_FP64) {
+    if constexpr (MODE != LYA_MODE_MIXED_F32) {
+        // FP64 sums: the pixel weights are read as FP64
         double w1[PPL], d1[PPL];
 #pragma unroll
         for (int k = 0; k < PPL; ++k) {
@@ -171,38 +188,118 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
         const CAS double* r2p = cptr(p.dc) + p2;
         const CAS double* w2p = cptr(p.we) + p2;
         const CAS double* d2p = cptr(p.dw) + p2;
-        auto block = [&](int j, auto UC) {
-            constexpr int UU = decltype(UC)::value;
-            double r2[UU], w2[UU], d2[UU];
-#pragma unroll
-            for (int u = 0; u < UU; ++u) {
-                r2[u] = r2p[j + u];
-                w2[u] = w2p[j + u];
-                d2[u] = d2p[j + u];
-            }
-#pragma unroll
-            for (int u = 0; u < UU; ++u) {
-#pragma unroll
-                for (int k = 0; k < PPL; ++k) {
-                    const double rp = fabs(r1[k] - r2[u]) * c;
-                    const double rt = (r1[k] + r2[u]) * s;
-                    const uint32_t bp = (uint32_t)__double2int_rz(rp * bp64);
-                    const uint32_t bt = (uint32_t)__double2int_rz(rt * bt64);
-                    const double pw = w1[k] * w2[u], pd = d1[k] * d2[u];
-                    if (bp < (uint32_t)NP && bt < (uint32_t)NT) {
-                        const uint32_t a4 = __umul24(bp, ntpad4) + (bt << 2);
-                        lds_add<uint32_t>(lds, hbase + a4, 1u);
-                        lds_add<double>(lds, hbase + L::W + 2u * a4, pw);
-                        lds_add<double>(lds, hbase + L::DW + 2u * a4, pd);
-                    }
+        auto accumulate = [&](uint32_t bp, uint32_t bt, double pw, double pd) {
+            if (bp < (uint32_t)NP && bt < (uint32_t)NT) {
+                if constexpr (CNT) {
+                    const uint32_t a4 = __umul24(bp, ntpad4) + opaque((bt << 2) + hbase);
+                    lds_add<uint32_t>(lds, a4, 1u);
+                    const uint32_t a8 = (a4 << 1) + woff;
+                    lds_add<double>(lds, a8, pw);
+                    lds_add<double>(lds, a8 + (L::DW - L::W), pd);
+                } else {
+                    const uint32_t a8 = __umul24(bp, ntpad8) + opaque((bt << 3) + hbw);
+                    lds_add<double>(lds, a8, pw);
+                    lds_add<double>(lds, a8 + (L::DW - L::W), pd);
                 }
             }
         };
-        constexpr int U = 4;
-        int j = jlo;
-        for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
-        for (; j < jhi; ++j) block(j, std::integral_constant<int, 1>{});
+        if constexpr (MODE == LYA_MODE_FP64) {
+            const double cq = POW2 ? c * bp64 : c, sq = POW2 ? s * bt64 : s;
+            auto block = [&](int j, auto UC) {
+                constexpr int UU = decltype(UC)::value;
+                double r2[UU], w2[UU], d2[UU];
+#pragma unroll
+                for (int u = 0; u < UU; ++u) {
+                    r2[u] = r2p[j + u];
+                    w2[u] = w2p[j + u];
+                    d2[u] = d2p[j + u];
+                }
+#pragma unroll
+                for (int u = 0; u < UU; ++u) {
+#pragma unroll
+                    for (int k = 0; k < PPL; ++k) {
+                        double qp = fabs(r1[k] - r2[u]) * cq;
+                        double qt = (r1[k] + r2[u]) * sq;
+                        if constexpr (!POW2) {
+                            qp = qp * bp64;
+                            qt = qt * bt64;
+                        }
+                        accumulate((uint32_t)__double2int_rz(qp), (uint32_t)__double2int_rz(qt), w1[k] * w2[u],
+                                   d1[k] * d2[u]);
+                    }
+                }
+            };
+            constexpr int U = 4;
+            int j = jlo;
+            for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
+            for (; j < jhi; ++j) block(j, std::integral_constant<int, 1>{});
+        } else {
+            // FP32 geometry, FP64 recheck near edges, exact FP64 sums
+            const double base2 = cptr(p.base)[f2];
+            const double cb = c * bp64, sb = s * bt64;
+            const float c4 = unif((float)cb);
+            const float nc4 = -c4;
+            const float s4 = unif((float)sb);
+            const float tau = p.tau;
+            const float tau2 = 2.0f * tau;
+            float P1[PPL], T1[PPL];
+#pragma unroll
+            for (int k = 0; k < PPL; ++k) {
+                P1[k] = valid[k] ? (float)((r1[k] - base2) * cb) : 1048576.5f;  // 2^20 + 1/2: out of range
+                T1[k] = valid[k] ? (float)((r1[k] + base2) * sb + (double)tau) : 1048576.5f;
+            }
+            const CAS float* o2p = cptr(p.off32) + p2;
+            auto block = [&](int j, auto UC) {
+                constexpr int UU = decltype(UC)::value;
+                uint32_t bp[UU][PPL], bt[UU][PPL];
+                float fr[UU][PPL];
+                float o2v[UU];
+                double w2[UU], d2[UU];
+                float frmin = 1.0f;
+#pragma unroll
+                for (int u = 0; u < UU; ++u) {
+                    o2v[u] = o2p[j + u];
+                    w2[u] = w2p[j + u];
+                    d2[u] = d2p[j + u];
+                }
+#pragma unroll
+                for (int u = 0; u < UU; ++u) {
+#pragma unroll
+                    for (int k = 0; k < PPL; ++k) {
+                        const float qp = __builtin_fabsf(__builtin_fmaf(o2v[u], nc4, P1[k])) + tau;
+                        const float qt = __builtin_fmaf(o2v[u], s4, T1[k]);
+                        bp[u][k] = __float2uint_rz(qp);
+                        bt[u][k] = __float2uint_rz(qt);
+                        fr[u][k] = fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt));
+                        frmin = fminf(frmin, fr[u][k]);
+                    }
+                }
+                if (frmin < tau2) {
+#pragma unroll
+                    for (int u = 0; u < UU; ++u) {
+#pragma unroll
+                        for (int k = 0; k < PPL; ++k) {
+                            if (fr[u][k] < tau2) {
+                                const double r2 = r2p[j + u];
+                                bp[u][k] = (uint32_t)__double2int_rz((fabs(r1[k] - r2) * c) * bp64);
+                                bt[u][k] = (uint32_t)__double2int_rz(((r1[k] + r2) * s) * bt64);
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < UU; ++u) {
+#pragma unroll
+                    for (int k = 0; k < PPL; ++k) accumulate(bp[u][k], bt[u][k], w1[k] * w2[u], d1[k] * d2[u]);
+                }
+            };
+            constexpr int U = 8;
+            int j = jlo;
+            for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
+            for (; j < jhi; ++j) block(j, std::integral_constant<int, 1>{});
+        }
     } else {
+        // FP32 geometry, FP32 products and FP32 LDS partials
         const double base2 = cptr(p.base)[f2];
         const double cb = c * bp64, sb = s * bt64;
         const float c4 = unif((float)cb);
@@ -219,7 +316,7 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                 w1[k] = p.we32[pix1[k]];
                 d1[k] = p.dw32[pix1[k]];
             } else {
-                P1[k] = 1048576.5f;  // 2^20 + 1/2: far out of range, fract = 1/2
+                P1[k] = 1048576.5f;
                 T1[k] = 1048576.5f;
                 w1[k] = 0.f;
                 d1[k] = 0.f;
@@ -244,12 +341,10 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
             }
 #pragma unroll
             for (int u = 0; u < UU; ++u) {
-                const float o2 = o2v[u];
 #pragma unroll
                 for (int k = 0; k < PPL; ++k) {
-                    const float v = __builtin_fmaf(o2, nc4, P1[k]);
-                    const float qp = __builtin_fabsf(v) + tau;
-                    const float qt = __builtin_fmaf(o2, s4, T1[k]);
+                    const float qp = __builtin_fabsf(__builtin_fmaf(o2v[u], nc4, P1[k])) + tau;
+                    const float qt = __builtin_fmaf(o2v[u], s4, T1[k]);
                     bp[u][k] = __float2uint_rz(qp);
                     bt[u][k] = __float2uint_rz(qt);
                     fr[u][k] = fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt));
@@ -257,46 +352,28 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                 }
             }
             if (frmin < tau2) {
-                // near a bin edge somewhere in the block: exact FP64 bins for those pairs
 #pragma unroll
                 for (int u = 0; u < UU; ++u) {
 #pragma unroll
                     for (int k = 0; k < PPL; ++k) {
                         if (fr[u][k] < tau2) {
                             const double r2 = r2p[j + u];
-                            const double rp = fabs(r1[k] - r2) * c;
-                            const double rt = (r1[k] + r2) * s;
-                            bp[u][k] = (uint32_t)__double2int_rz(rp * bp64);
-                            bt[u][k] = (uint32_t)__double2int_rz(rt * bt64);
+                            bp[u][k] = (uint32_t)__double2int_rz((fabs(r1[k] - r2) * c) * bp64);
+                            bt[u][k] = (uint32_t)__double2int_rz(((r1[k] + r2) * s) * bt64);
                         }
                     }
                 }
             }
 #pragma unroll
             for (int u = 0; u < UU; ++u) {
-                if constexpr (L::F32) {
 #pragma unroll
-                    for (int k = 0; k < PPL; ++k) {
-                        const float pw = w1[k] * w2v[u], pd = d1[k] * d2v[u];
-                        if (bp[u][k] < (uint32_t)NP && bt[u][k] < (uint32_t)NT) {
-                            const uint32_t a = __umul24(bp[u][k], ntpad4) + (bt[u][k] << 2) + hbase;
-                            lds_add<uint32_t>(lds, a, 1u);
-                            lds_add<float>(lds, a + L::W, pw);
-                            lds_add<float>(lds, a + L::DW, pd);
-                        }
-                    }
-                } else {
-                    // products of FP32 inputs are exact in FP64
-                    const double w2d = (double)w2v[u], d2d = (double)d2v[u];
-#pragma unroll
-                    for (int k = 0; k < PPL; ++k) {
-                        const double pw = (double)w1[k] * w2d, pd = (double)d1[k] * d2d;
-                        if (bp[u][k] < (uint32_t)NP && bt[u][k] < (uint32_t)NT) {
-                            const uint32_t a4 = __umul24(bp[u][k], ntpad4) + (bt[u][k] << 2);
-                            lds_add<uint32_t>(lds, hbase + a4, 1u);
-                            lds_add<double>(lds, hbase + L::W + 2u * a4, pw);
-                            lds_add<double>(lds, hbase + L::DW + 2u * a4, pd);
-                        }
+                for (int k = 0; k < PPL; ++k) {
+                    const float pw = w1[k] * w2v[u], pd = d1[k] * d2v[u];
+                    if (bp[u][k] < (uint32_t)NP && bt[u][k] < (uint32_t)NT) {
+                        const uint32_t a = __umul24(bp[u][k], ntpad4) + opaque((bt[u][k] << 2) + hbase);
+                        if constexpr (CNT) lds_add<uint32_t>(lds, a, 1u);
+                        lds_add<float>(lds, a + L::W, pw);
+                        lds_add<float>(lds, a + L::DW, pd);
                     }
                 }
             }
@@ -328,7 +405,7 @@ __device__ void flush_item(unsigned char* lds, int R, int nb_lds, int tid, int n
 }
 
 // End of tile: merge the workgroup's sums into the slot's FP64 global sums.
-template <int MODE>
+template <int MODE, bool CNT>
 __device__ void flush_tile(const lya_xi_problem& p, unsigned char* lds, int R, int ntpad, int slot, int tid,
                            int nthr) {
     using L = Layout<MODE>;
@@ -358,20 +435,21 @@ __device__ void flush_tile(const lya_xi_problem& p, unsigned char* lds, int R, i
             aw[b] = 0.0;
             ad[b] = 0.0;
         }
-        if (cnt) {
+        if (CNT ? (cnt != 0u) : (sw != 0.0 || sd != 0.0)) {
             const int bp = b / ntpad, bt = b - bp * ntpad;
             const int64_t g = gofs + (int64_t)bp * p.np_rt + bt;
-            atomicAdd((unsigned long long*)(p.out_cnt + g), (unsigned long long)cnt);
+            if (CNT) atomicAdd((unsigned long long*)(p.out_cnt + g), (unsigned long long)cnt);
             unsafeAtomicAdd(p.out_w + g, sw);
             unsafeAtomicAdd(p.out_dw + g, sd);
         }
     }
 }
 
-template <int MODE, int PPL, int WAVES>
-__global__ void __launch_bounds__(WAVES * 64) xi_kernel(const lya_xi_problem p, int R, int ntpad) {
+template <int MODE, int PPL, int WAVES, bool CNT, bool POW2, int R>
+__global__ void __launch_bounds__(WAVES * 64) xi_kernel(const lya_xi_problem p, int ntpad) {
     using L = Layout<MODE>;
-    extern __shared__ __align__(16) unsigned char lds[];
+    // statically sized, so histogram addresses fold into the ds offset field
+    __shared__ __align__(16) unsigned char lds[L::bytes(R)];
     __shared__ int s_tile;
     constexpr int NTHR = WAVES * 64;
     const int tid = threadIdx.x;
@@ -410,15 +488,18 @@ __global__ void __launch_bounds__(WAVES * 64) xi_kernel(const lya_xi_problem p, 
                 const int q = k / nslab;
                 const int slab = k - q * nslab;
                 const int f2 = cptr(p.nb_idx)[nb0 + q];
-                run_task<MODE, PPL>(p, lds, hbase, ntpad4, t, slab, f2, lane, perm, bp64, bt64);
+                run_task<MODE, PPL, CNT, POW2>(p, lds, hbase, ntpad4, t, slab, f2, lane, perm, bp64, bt64);
             }
             if constexpr (MODE == LYA_MODE_MIXED_F32) {
+                // every wave done with the item; then the partials move to FP64 and are
+                // zeroed before any wave starts the next item
                 __syncthreads();
                 flush_item(lds, R, nb_lds, tid, NTHR);
+                __syncthreads();
             }
         }
         __syncthreads();
-        flush_tile<MODE>(p, lds, R, ntpad, slot, tid, NTHR);
+        flush_tile<MODE, CNT>(p, lds, R, ntpad, slot, tid, NTHR);
         __syncthreads();
     }
 }
@@ -512,8 +593,8 @@ __global__ void __launch_bounds__(WAVES * 64) xi_adjoint_kernel(const lya_xi_pro
                         }
                         if (bp < (uint32_t)NP && bt < (uint32_t)NT) {
                             const int A = bp * NT + bt;
-                            gd += ta[A] * (double)cptr(p.dw32)[p2 + j];
-                            gw += tb[A] * (double)cptr(p.we32)[p2 + j];
+                            gd += ta[A] * cptr(p.dw)[p2 + j];
+                            gw += tb[A] * cptr(p.we)[p2 + j];
                         }
                     }
                 }
@@ -551,21 +632,31 @@ __global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ d
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-using XiKernel = void (*)(const lya_xi_problem, int, int);
+using XiKernel = void (*)(const lya_xi_problem, int);
 
-template <int MODE, int PPL>
+template <int MODE, int PPL, bool CNT, bool POW2, int R>
 XiKernel pick_w(int waves) {
-    if (waves == 4) return xi_kernel<MODE, PPL, 4>;
-    if (waves == 8) return xi_kernel<MODE, PPL, 8>;
+    if (waves == 4) return xi_kernel<MODE, PPL, 4, CNT, POW2, R>;
+    if (waves == 8) return xi_kernel<MODE, PPL, 8, CNT, POW2, R>;
     return nullptr;
 }
-template <int MODE>
+template <int MODE, bool CNT, bool POW2, int R>
 XiKernel pick_p(int ppl, int waves) {
-    static_assert(MODE == LYA_MODE_FP64 || MODE == LYA_MODE_MIXED || MODE == LYA_MODE_MIXED_F32, "mode");
-    if (ppl == 1) return pick_w<MODE, 1>(waves);
-    if (ppl == 2) return pick_w<MODE, 2>(waves);
-    if (ppl == 4) return pick_w<MODE, 4>(waves);
+    if (ppl == 1) return pick_w<MODE, 1, CNT, POW2, R>(waves);
+    if (ppl == 2) return pick_w<MODE, 2, CNT, POW2, R>(waves);
+    if (ppl == 4) return pick_w<MODE, 4, CNT, POW2, R>(waves);
     return nullptr;
+}
+template <int MODE, bool POW2>
+XiKernel pick_c(bool cnt, int R, int ppl, int waves) {
+    if (R == 1) return cnt ? pick_p<MODE, true, POW2, 1>(ppl, waves) : pick_p<MODE, false, POW2, 1>(ppl, waves);
+    if (R == 2) return cnt ? pick_p<MODE, true, POW2, 2>(ppl, waves) : pick_p<MODE, false, POW2, 2>(ppl, waves);
+    return nullptr;
+}
+
+bool is_pow2(double x) {
+    int e = 0;
+    return x > 0 && std::frexp(x, &e) == 0.5;
 }
 
 struct Resolved {
@@ -582,11 +673,17 @@ int resolve(const lya_xi_problem* p, const lya_xi_config* cfg, Resolved& r) {
     const int pad = (cfg && cfg->row_pad > 0) ? cfg->row_pad : 0;
     if (r.mode != LYA_MODE_FP64 && r.mode != LYA_MODE_MIXED && r.mode != LYA_MODE_MIXED_F32)
         return lya_fail(LYA_ERR_ARG, "unknown mode %d", r.mode);
-    if (r.R != 1 && r.R != 2 && r.R != 4) return lya_fail(LYA_ERR_ARG, "replicas must be 1, 2 or 4");
+    if (r.R != 1 && r.R != 2) return lya_fail(LYA_ERR_ARG, "replicas must be 1 or 2");
     if (r.R > r.waves) return lya_fail(LYA_ERR_ARG, "replicas > waves");
-    r.fn = (r.mode == LYA_MODE_FP64)    ? pick_p<LYA_MODE_FP64>(r.ppl, r.waves)
-           : (r.mode == LYA_MODE_MIXED) ? pick_p<LYA_MODE_MIXED>(r.ppl, r.waves)
-                                        : pick_p<LYA_MODE_MIXED_F32>(r.ppl, r.waves);
+    const bool cnt = !(cfg && cfg->no_counts);
+    const bool pow2 = p && is_pow2((double)p->np_rp / p->rpmax) && is_pow2((double)p->np_rt / p->rtmax);
+    if (r.mode == LYA_MODE_FP64)
+        r.fn = pow2 ? pick_c<LYA_MODE_FP64, true>(cnt, r.R, r.ppl, r.waves)
+                    : pick_c<LYA_MODE_FP64, false>(cnt, r.R, r.ppl, r.waves);
+    else if (r.mode == LYA_MODE_MIXED)
+        r.fn = pick_c<LYA_MODE_MIXED, false>(cnt, r.R, r.ppl, r.waves);
+    else
+        r.fn = pick_c<LYA_MODE_MIXED_F32, false>(cnt, r.R, r.ppl, r.waves);
     if (!r.fn) return lya_fail(LYA_ERR_ARG, "unsupported shape px_per_lane=%d waves=%d", r.ppl, r.waves);
     if (p) {
         if (p->np_rp < 1 || p->np_rt < 1 || !(p->rpmax > 0) || !(p->rtmax > 0))
@@ -598,16 +695,16 @@ int resolve(const lya_xi_problem* p, const lya_xi_config* cfg, Resolved& r) {
     } else {
         r.ntpad = 0;
     }
-    r.lds = (r.mode == LYA_MODE_MIXED_F32) ? Layout<LYA_MODE_MIXED_F32>::bytes(r.R) : Layout<LYA_MODE_FP64>::bytes(r.R);
+    r.lds = 16 + ((r.mode == LYA_MODE_MIXED_F32) ? Layout<LYA_MODE_MIXED_F32>::bytes(r.R)
+                                                 : Layout<LYA_MODE_FP64>::bytes(r.R));  // static LDS
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
         return lya_fail(LYA_ERR_HIP, "no HIP device");
     if (r.lds > prop.sharedMemPerBlock)
         return lya_fail(LYA_ERR_UNSUPPORTED, "LDS %u bytes > %zu per workgroup", r.lds, prop.sharedMemPerBlock);
-    hipFuncSetAttribute((const void*)r.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)r.lds);
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)r.fn, r.waves * 64, r.lds);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)r.fn, r.waves * 64, 0);
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     r.groups = (cfg && cfg->groups > 0) ? cfg->groups : per_cu * prop.multiProcessorCount;
     return LYA_OK;
@@ -618,9 +715,9 @@ int check_problem(const lya_xi_problem* p, int mode) {
     if (!p->pix_off || !p->ux || !p->uy || !p->uz || !p->sorted || !p->dc || !p->nb_idx || !p->items ||
         !p->tiles || !p->queue)
         return lya_fail(LYA_ERR_ARG, "null device pointer in problem");
-    if (mode == LYA_MODE_FP64 && (!p->we || !p->dw)) return lya_fail(LYA_ERR_ARG, "FP64 mode needs we, dw");
-    if (mode != LYA_MODE_FP64 && (!p->off32 || !p->we32 || !p->dw32 || !p->base))
-        return lya_fail(LYA_ERR_ARG, "mixed mode needs off32, we32, dw32, base");
+    if (mode != LYA_MODE_MIXED_F32 && (!p->we || !p->dw)) return lya_fail(LYA_ERR_ARG, "mode needs we, dw (FP64)");
+    if (mode != LYA_MODE_FP64 && (!p->off32 || !p->base)) return lya_fail(LYA_ERR_ARG, "mixed modes need off32, base");
+    if (mode == LYA_MODE_MIXED_F32 && (!p->we32 || !p->dw32)) return lya_fail(LYA_ERR_ARG, "mixed32 needs we32, dw32");
     if (mode != LYA_MODE_FP64 && !(p->tau > 0.0f && p->tau < 0.25f))
         return lya_fail(LYA_ERR_ARG, "mixed mode needs 0 < tau < 0.25 (got %g)", (double)p->tau);
     if (p->n_tiles < 0 || p->n_items < 0 || p->n_slots < 0) return lya_fail(LYA_ERR_ARG, "negative sizes");
@@ -634,7 +731,7 @@ int launch_all(const lya_xi_problem* p, const Resolved& r, hipStream_t st) {
     hipLaunchKernelGGL(zero_outputs, dim3(zb), dim3(256), 0, st, p->out_cnt, p->out_w, p->out_dw, n, p->queue);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return lya_fail(LYA_ERR_HIP, "zero_outputs launch: %s", hipGetErrorString(e));
-    hipLaunchKernelGGL(r.fn, dim3(r.groups), dim3(r.waves * 64), r.lds, st, *p, r.R, r.ntpad);
+    hipLaunchKernelGGL(r.fn, dim3(r.groups), dim3(r.waves * 64), 0, st, *p, r.ntpad);
     e = hipGetLastError();
     if (e != hipSuccess) return lya_fail(LYA_ERR_HIP, "xi kernel launch: %s", hipGetErrorString(e));
     return LYA_OK;

@@ -71,6 +71,7 @@ def default_config(mode="mixed", **kw):
     cfg.replicas = kw.get("replicas", 1)
     cfg.row_pad = kw.get("row_pad", 1)
     cfg.groups = kw.get("groups", 0)
+    cfg.no_counts = 0 if kw.get("counts", True) else 1
     return cfg
 
 

@@ -59,6 +59,21 @@ def test_kernel_shapes(small_set, small_pairs):
         assert np.array_equal(c, c0), (mode, ppl, waves, rep, pad)
 
 
+def test_without_counts(small_set, small_pairs):
+    """Counts off: identical sums, out_cnt left zero."""
+    am, (off, idx) = small_pairs
+    eng = xi.XiEngine(small_set, am, pairs=(off, idx))
+    for mode in ("fp64", "mixed", "mixed32"):
+        eng.run(xi.default_config(mode))
+        _, w0, d0 = eng.results()
+        eng.run(xi.default_config(mode, counts=False))
+        c, w, d = eng.results()
+        assert c.sum() == 0
+        tol = 1e-5 if mode == "mixed32" else 1e-12
+        np.testing.assert_allclose(w, w0, rtol=tol)
+        assert np.all(np.abs(d - d0) <= tol * np.abs(w0) + 1e-300)
+
+
 def test_adversarial_edges_bit_exact():
     """Pixel pairs within 1e-6 Mpc/h of bin edges: mixed bins equal FP64 bins."""
     s = synth.adversarial(2000, seed=1002)
