@@ -77,12 +77,77 @@ def cpu_baseline(fs, eng, angmax, budget_pairs, threads):
                       % (m, fs.nf, P, dt)}
 
 
+def bench_c4(args):
+    """Distortion matrix (5% subsample, FP64 accumulation) and one chi^2 gradient on
+    20k forests: wall times of the device work (reported, not the headline)."""
+    import random
+    import numpy as np
+    import torch
+    from lya2pcf_amd import adjoint, forests, synth
+    from lya2pcf_amd import distortion as dist
+    cfg = synth.CONFIGS["c4"]
+    if args.forests:
+        cfg = synth.scaled(cfg, args.forests)
+    s = synth.generate(cfg)
+    fs = forests.ForestSet.from_survey(s, float32_inputs=True)
+    am = forests.angmax_from(fs, 200.0)
+    eng = dist.DistortionEngine(fs, am)
+    pairs = dist.subsample(fs, eng.nb_off, eng.nb_idx, range(fs.nf), 0.95, rng=random.Random(1))
+    n = np.diff(fs.pix_off)
+    P = float((n[pairs[:, 0]] * n[pairs[:, 1]]).sum())
+    times = []
+    for k in range(args.warmup + args.steps):
+        eng.zero()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.accumulate(pairs)
+        torch.cuda.synchronize()
+        if k >= args.warmup:
+            times.append(time.perf_counter() - t0)
+    D, wA = eng.result()
+    td = float(np.median(times))
+    log("distortion: %d subsampled forest pairs, %.4g pixel pairs, %.3f s" % (len(pairs), P, td))
+    # chi^2 gradient: forward pair sums + two adjoint walks
+    ae = adjoint.AdjointEngine(fs, am, mode="mixed", pairs=(eng.nb_off, eng.nb_idx))
+    dev = "cuda"
+    wm = adjoint.WeightModel(s.z, s.sigma2_pip, cfg.z_nodes, dev)
+    pr = adjoint.Projection(s.pix_off, s.loglam, dev)
+    m = torch.from_numpy(adjoint.model_xi()).to(dev)
+    delta = torch.from_numpy(s.delta.astype(np.float32).astype(np.float64)).to(dev)
+    tg = []
+    for k in range(args.warmup + args.steps):
+        eta = torch.tensor(1.0, dtype=torch.float64, device=dev, requires_grad=True)
+        s2 = torch.full((10,), 0.05, dtype=torch.float64, device=dev, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        w = wm(eta, s2)
+        W, DW = adjoint.pair_sums(pr(delta, w), w, ae)
+        adjoint.chi2(W, DW, m, 1e-4).backward()
+        torch.cuda.synchronize()
+        if k >= args.warmup:
+            tg.append(time.perf_counter() - t0)
+    tgm = float(np.median(tg))
+    P2 = ae.fwd.pixel_pairs
+    log("chi2 gradient: %.3f s for %.4g pixel pairs (forward + 2 adjoint walks)" % (tgm, P2))
+    print(json.dumps({"metric": "distortion matrix wall time (5% subsample, FP64)", "value": td, "unit": "s",
+                      "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "data": "synthetic", "dtype": "f64",
+                      "config": {"workload": cfg.name, "forests": fs.nf, "subsampled_pairs": int(len(pairs)),
+                                 "pixel_pairs": P},
+                      "pixel_pairs_per_s": P / td,
+                      "rows_with_weight": int((wA > 0).sum()),
+                      "chi2_gradient": {"seconds": tgm, "pixel_pairs": P2, "pixel_pairs_per_s": 3 * P2 / tgm,
+                                        "note": "one forward and two adjoint walks of the full pair list"}}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c3"])
+    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c3", "c4"],
+                    help="c1 (default): the xi headline; c0/c3: xi on those surveys; c4: distortion + gradient")
     ap.add_argument("--forests", type=int, default=0, help="override the number of forests (same density)")
     ap.add_argument("--cpu-pairs", type=float, default=1.5e11, help="CPU baseline sample size in pixel pairs")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -109,6 +174,8 @@ def main():
     from lya2pcf_amd import autotune, forests, synth, xi
 
     torch.cuda.init()
+    if args.config == "c4":
+        return bench_c4(args)
     cfgname = args.config
     cfg = synth.CONFIGS[cfgname]
     if args.forests:

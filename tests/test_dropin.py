@@ -1,0 +1,119 @@
+"""Drop-in contract: data1.npy in, per-pixel .npy files out, read the way
+post-processing reads them (reference 2pla.py:52-140, distortion.py:48-125,
+post_processing.py:130-147)."""
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from lya2pcf_amd import forests, synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _workspace(tmp_path, n=150, seed=7, **kw):
+    """tmp dir with the drivers, a parameters.py pointing at tmp paths, and data1.npy."""
+    for f in ("2pla.py", "distortion.py", "correlation_procedures_hip.py", "distortion_procedures_hip.py",
+              "forest_class.py"):
+        shutil.copy(os.path.join(ROOT, f), tmp_path / f)
+    par = open(os.path.join(ROOT, "parameters.py")).read()
+    par = par.replace("data_dir = './deltas_lya2pcf/'", "data_dir = '%s/deltas/'" % tmp_path)
+    par = par.replace("corr_dir = './outputs/test/'", "corr_dir = '%s/out/'" % tmp_path)
+    (tmp_path / "parameters.py").write_text(par)
+    (tmp_path / "deltas").mkdir()
+    s = synth.generate(synth.scaled(synth.CONFIGS["c0"], n, seed=seed, **kw))   # nside 32, as parameters.py
+    synth.write_data_npy(s, str(tmp_path / "deltas" / "data1.npy"))
+    return s
+
+
+def _run(tmp_path, *args):
+    env = dict(os.environ, PYTHONPATH=ROOT)
+    r = subprocess.run([sys.executable, str(tmp_path / args[0]), *args[1:]], cwd=tmp_path, env=env,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r
+
+
+def test_data_npy_round_trip(tmp_path):
+    """data1.npy (pickled {pixel: [quasar]}) packs to the same arrays as the survey."""
+    s = synth.generate(synth.scaled(synth.CONFIGS["c0"], 40, seed=3))
+    path = tmp_path / "data1.npy"
+    synth.write_data_npy(s, str(path))
+    data = np.load(path, allow_pickle=True).item()
+    assert sorted(data.keys()) == list(data.keys())
+    fa = forests.ForestSet.from_data(data)
+    fb = forests.ForestSet.from_survey(s)
+    for k in ("pix_off", "dc", "we", "dw", "ux", "uy", "uz", "ra", "name"):
+        assert np.array_equal(getattr(fa, k), getattr(fb, k)), k
+    q = data[next(iter(data))][0]
+    for attr in ("ra", "dec", "name", "pix", "x", "y", "z", "dc", "we", "dw", "rx", "delta_lambda", "omega",
+                 "omega_delta_lambda2", "lenght", "fib", "pl"):
+        assert hasattr(q, attr), attr
+    from forest_class import quasar
+    from parameters import nside
+    from lya2pcf_amd import healpix
+    assert isinstance(q, quasar)
+    assert q.pix == healpix.ang2pix_ring(nside, np.pi / 2 - q.dec, q.ra)
+
+
+def test_quasar_neighborhood_matches_pair_list():
+    """forest_class.quasar.neighborhood (healpy-free) gives the native pair list."""
+    from lya2pcf_amd import xi
+    s = synth.generate(synth.scaled(synth.CONFIGS["c0"], 120, seed=5))
+    data = synth.to_data_dict(s)
+    fs = forests.ForestSet.from_data(data)
+    am = forests.angmax_from(fs, 200.0)
+    off, idx = xi.build_pairs(fs, am)
+    pos = {id(q): k for k, q in enumerate(fs.objects)}
+    for k, q in enumerate(fs.objects):
+        got = sorted(pos[id(f)] for f in q.neighborhood(data, am))
+        assert got == list(idx[off[k]:off[k + 1]]), k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_2pla_gpu_outputs(tmp_path, precision):
+    s = _workspace(tmp_path)
+    _run(tmp_path, "2pla.py", "--gpu", "--counts", "--precision", precision)
+    out = tmp_path / "out"
+    fs = forests.ForestSet.from_survey(s)
+    am = forests.angmax_from(fs, 200.0)
+    off, idx = oracle.pairs(fs, am)
+    c0, w0, d0 = oracle.xi_gpu_form(fs, off, idx)
+    files = sorted(out.glob("2d_histogram_pixel_*"))
+    assert len(files) == len(fs.slots)
+    for k, pix in enumerate(fs.slots):
+        h = np.load(out / ("2d_histogram_pixel_%d.npy" % pix))
+        assert h.shape == (2, 50, 50) and h.dtype == np.float64
+        # post_processing.py:139 unpacks exactly two arrays
+        w_hist, dw_hist = h
+        cnt = np.load(out / ("2d_paircounts_pixel_%d.npy" % pix))
+        assert np.array_equal(cnt, c0[k])
+        np.testing.assert_allclose(w_hist, w0[k], rtol=1e-12)
+        np.testing.assert_allclose(dw_hist, d0[k], rtol=1e-9, atol=1e-9 * np.abs(w0[k]).max())
+    assert (out / "thread_0_of_1.log").exists()
+
+
+@pytest.mark.gpu
+def test_distortion_driver(tmp_path):
+    import random
+    from lya2pcf_amd import distortion as dist
+    s = _workspace(tmp_path, n=60, seed=8, length_range=(30, 60))
+    _run(tmp_path, "distortion.py", "--excluded", "0.8")
+    D = np.load(tmp_path / "out" / "distortion.npy")
+    assert D.shape == (2500, 2500) and D.dtype == np.float64
+    fs = forests.ForestSet.from_survey(s)
+    am = forests.angmax_from(fs, 200.0)
+    off, idx = oracle.pairs(fs, am)
+    pairs = dist.subsample(fs, off, idx, range(fs.nf), 0.8, rng=random.Random(1))
+    D0, w0 = oracle.distortion(fs, pairs)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        E = D0 / w0[:, None]
+    ok = w0 > 0
+    assert np.array_equal(np.isfinite(D).all(1), ok)
+    scale = np.abs(E[ok]).max()
+    assert np.max(np.abs(D[ok] - E[ok])) <= 1e-10 * scale
