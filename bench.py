@@ -195,7 +195,8 @@ def main():
 
     def shape_for(mode):
         best = autotune.load_best(mode)
-        shape = {k: v for k, v in (best or {}).items() if k in ("px_per_lane", "waves", "replicas", "row_pad")}
+        shape = {k: v for k, v in (best or {}).items()
+                 if k in ("px_per_lane", "waves", "replicas", "row_pad", "lane_map")}
         for k, a in (("px_per_lane", args.ppl), ("waves", args.waves), ("replicas", args.replicas)):
             if a:
                 shape[k] = a

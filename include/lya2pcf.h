@@ -100,7 +100,7 @@ typedef struct lya_xi_problem {
     double rpmax;
     double rtmax;
     float tau;                /* mixed: half-width of the FP64 recheck band, bin units */
-    int32_t pad0;
+    int32_t lane_map;         /* lane -> pixel order: 0 consecutive, 1 by 2, 2 by 4   */
     uint64_t* out_cnt;        /* [n_slots][np_rp*np_rt] pair counts                   */
     double* out_w;            /* [n_slots][np_rp*np_rt] sum w1*w2                     */
     double* out_dw;           /* [n_slots][np_rp*np_rt] sum dw1*dw2                   */
@@ -111,8 +111,8 @@ typedef struct lya_xi_problem {
 typedef struct lya_xi_config {
     int32_t mode;             /* LYA_MODE_FP64, LYA_MODE_MIXED or LYA_MODE_MIXED_F32  */
     int32_t px_per_lane;      /* 1, 2, 4: forest-1 tile of 64, 128, 256 pixels        */
-    int32_t waves;            /* waves per workgroup: 4 or 8                          */
-    int32_t replicas;         /* LDS histogram copies: 1, 2, 4                        */
+    int32_t waves;            /* waves per workgroup: 4, 8 or 16 (replicas 2: 8 only) */
+    int32_t replicas;         /* LDS histogram copies: 1 or 2                         */
     int32_t row_pad;          /* extra LDS columns per r_parallel row: 0 or 1         */
     int32_t groups;           /* workgroups in the persistent grid (0 = fill the GPU)  */
     int32_t no_counts;        /* 1: skip the per-bin pair counts (out_cnt left zero)  */

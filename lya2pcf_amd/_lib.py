@@ -7,7 +7,8 @@ missing: there is no CPU fallback behind these entry points.
 import ctypes as C
 import os
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build", "liblya2pcf.so")
+LIB_PATH = os.environ.get("LYA2PCF_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build",
+                                                         "liblya2pcf.so")
 
 LYA_MODE_FP64 = 0
 LYA_MODE_MIXED = 1
@@ -28,15 +29,17 @@ class XiProblem(C.Structure):
         ("dc", _vp), ("we", _vp), ("dw", _vp), ("off32", _vp), ("we32", _vp), ("dw32", _vp),
         ("nb_idx", _vp), ("items", _vp), ("tiles", _vp),
         ("n_items", _i32), ("n_tiles", _i32), ("np_rp", _i32), ("np_rt", _i32),
-        ("rpmax", _f64), ("rtmax", _f64), ("tau", C.c_float), ("pad0", _i32),
+        ("rpmax", _f64), ("rtmax", _f64), ("tau", C.c_float), ("lane_map", _i32),
         ("out_cnt", _vp), ("out_w", _vp), ("out_dw", _vp), ("queue", _vp),
     ]
 
 
 class XiConfig(C.Structure):
-    """Mirror of ``lya_xi_config``."""
+    """Mirror of ``lya_xi_config`` (``lane_map`` is a Python-side field copied into
+    the problem's ``lane_map``)."""
     _fields_ = [("mode", _i32), ("px_per_lane", _i32), ("waves", _i32), ("replicas", _i32),
                 ("row_pad", _i32), ("groups", _i32), ("no_counts", _i32)]
+    lane_map = 1
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

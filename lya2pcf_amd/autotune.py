@@ -4,7 +4,10 @@ Search space (all shapes give identical bins):
   px_per_lane  1, 2, 4   -> forest-1 tile of 64, 128, 256 pixels per wave
   replicas     1, 2      -> LDS histogram copies (waves w, w+R, ... share one)
   row_pad      0, 1      -> LDS row stride np_rt or np_rt + 1 (bank padding)
-  waves        4, 8      -> waves per workgroup
+  waves        4, 8, 16  -> waves per workgroup
+  lane_map     0, 1, 2   -> lane order of forest-1 pixels (consecutive, by 2, by 4)
+The default sweep fixes replicas=1 and row_pad=1 (a second LDS replica halves
+occupancy; the padding removes bank conflicts: both measured in a full sweep).
 Each candidate is timed as hipGraph replays of the whole problem; the best
 shape per mode is written to ``tuned.json`` next to this file and used as the
 default by the benchmark.
@@ -21,7 +24,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuned.json")
 PROFILES = os.path.join(os.path.dirname(HERE), "profiles")
 
-SPACE = {"px_per_lane": (1, 2, 4), "replicas": (1, 2), "row_pad": (0, 1), "waves": (4, 8)}
+SPACE = {"px_per_lane": (1, 2, 4), "replicas": (1,), "row_pad": (1,), "waves": (8, 16), "lane_map": (0, 1, 2)}
 
 
 def load_best(mode="mixed"):
@@ -42,11 +45,14 @@ def measured_traffic(mode="mixed"):
         return None
 
 
-def candidates():
-    for ppl, rep, pad, waves in itertools.product(*SPACE.values()):
-        if rep > waves:
+def candidates(space=None):
+    space = space or SPACE
+    keys = list(space)
+    for vals in itertools.product(*space.values()):
+        c = dict(zip(keys, vals))
+        if c.get("replicas", 1) > c.get("waves", 8):
             continue
-        yield {"px_per_lane": ppl, "replicas": rep, "row_pad": pad, "waves": waves}
+        yield c
 
 
 def tune(eng, mode, reps=3, log=print):

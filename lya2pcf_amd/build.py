@@ -34,7 +34,11 @@ def _run(cmd):
     return r
 
 
-def build(verbose=False, extra_hip_flags=()):
+def build(verbose=False, extra_hip_flags=(), out_dir=None):
+    """Compile and link; ``out_dir`` (default lya2pcf_amd/_build) receives the objects and the .so."""
+    global OUT, LIB
+    if out_dir:
+        OUT, LIB = out_dir, os.path.join(out_dir, "liblya2pcf.so")
     os.makedirs(OUT, exist_ok=True)
     objs = []
     for src in HIP_SOURCES:

@@ -45,6 +45,14 @@
 
 #define CAS __attribute__((address_space(4)))
 
+// forest-2 pixels per scalar-load block
+#ifndef LYA_UNROLL_FP64
+#define LYA_UNROLL_FP64 4
+#endif
+#ifndef LYA_UNROLL_MIXED
+#define LYA_UNROLL_MIXED 8
+#endif
+
 namespace {
 
 constexpr int HS = LYA_MAX_BINS;  // LDS histogram entries per replica
@@ -229,7 +237,7 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                     }
                 }
             };
-            constexpr int U = 4;
+            constexpr int U = LYA_UNROLL_FP64;
             int j = jlo;
             for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
             for (; j < jhi; ++j) block(j, std::integral_constant<int, 1>{});
@@ -293,7 +301,7 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                     for (int k = 0; k < PPL; ++k) accumulate(bp[u][k], bt[u][k], w1[k] * w2[u], d1[k] * d2[u]);
                 }
             };
-            constexpr int U = 8;
+            constexpr int U = LYA_UNROLL_MIXED;
             int j = jlo;
             for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
             for (; j < jhi; ++j) block(j, std::integral_constant<int, 1>{});
@@ -455,7 +463,11 @@ __global__ void __launch_bounds__(WAVES * 64) xi_kernel(const lya_xi_problem p, 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = uni(tid >> 6);
-    const int perm = ((lane & 31) << 1) | (lane >> 5);
+    // lane -> forest-1 pixel within the 64-pixel row: 0 consecutive, 1 interleaved by 2
+    // (half-waves hold even / odd pixels), 2 interleaved by 4 (16-lane groups)
+    const int lmap = p.lane_map;
+    const int perm = lmap == 0 ? lane : (lmap == 1 ? (((lane & 31) << 1) | (lane >> 5))
+                                                   : (((lane & 15) << 2) | (lane >> 4)));
     const double bp64 = (double)p.np_rp / p.rpmax;
     const double bt64 = (double)p.np_rt / p.rtmax;
     const uint32_t ntpad4 = (uint32_t)ntpad * 4u;
@@ -636,7 +648,10 @@ using XiKernel = void (*)(const lya_xi_problem, int);
 
 template <int MODE, int PPL, bool CNT, bool POW2, int R>
 XiKernel pick_w(int waves) {
-    if (waves == 4) return xi_kernel<MODE, PPL, 4, CNT, POW2, R>;
+    if constexpr (R == 1) {
+        if (waves == 4) return xi_kernel<MODE, PPL, 4, CNT, POW2, 1>;
+        if (waves == 16) return xi_kernel<MODE, PPL, 16, CNT, POW2, 1>;
+    }
     if (waves == 8) return xi_kernel<MODE, PPL, 8, CNT, POW2, R>;
     return nullptr;
 }

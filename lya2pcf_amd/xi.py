@@ -72,6 +72,7 @@ def default_config(mode="mixed", **kw):
     cfg.row_pad = kw.get("row_pad", 1)
     cfg.groups = kw.get("groups", 0)
     cfg.no_counts = 0 if kw.get("counts", True) else 1
+    cfg.lane_map = kw.get("lane_map", 1)
     return cfg
 
 
@@ -171,6 +172,7 @@ class XiEngine:
         p.items, p.tiles, p.n_items, p.n_tiles = ptr(items), ptr(tiles), ni, nt
         p.np_rp, p.np_rt, p.rpmax, p.rtmax = self.np_rp, self.np_rt, self.rpmax, self.rtmax
         p.tau = self.tau
+        p.lane_map = getattr(cfg, "lane_map", 1)
         p.out_cnt, p.out_w, p.out_dw, p.queue = ptr(self.out_cnt), ptr(self.out_w), ptr(self.out_dw), ptr(self.queue)
         self._keep = (items, tiles)
         return p

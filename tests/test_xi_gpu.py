@@ -52,11 +52,13 @@ def test_kernel_shapes(small_set, small_pairs):
     am, (off, idx) = small_pairs
     c0, _, _ = oracle.xi_gpu_form(small_set, off, idx)
     eng = xi.XiEngine(small_set, am, pairs=(off, idx))
-    for mode, ppl, waves, rep, pad in itertools.product(["fp64", "mixed", "mixed32"], [1, 2, 4], [4, 8], [1, 2], [0, 1]):
-        cfg = xi.default_config(mode, px_per_lane=ppl, waves=waves, replicas=rep, row_pad=pad)
+    for mode, ppl, (waves, rep), pad, lmap in itertools.product(["fp64", "mixed", "mixed32"], [1, 2, 4],
+                                                                 [(4, 1), (8, 1), (8, 2), (16, 1)], [0, 1],
+                                                                 [0, 1, 2]):
+        cfg = xi.default_config(mode, px_per_lane=ppl, waves=waves, replicas=rep, row_pad=pad, lane_map=lmap)
         eng.run(cfg)
         c, _, _ = eng.results()
-        assert np.array_equal(c, c0), (mode, ppl, waves, rep, pad)
+        assert np.array_equal(c, c0), (mode, ppl, waves, rep, pad, lmap)
 
 
 def test_without_counts(small_set, small_pairs):
