@@ -11,6 +11,7 @@
  *
  * Reference interfaces replaced (Rafael9104/lya2pcf):
  *   lya_build_pairs     <- quasar.neighborhood          forest_class.py:78-98
+ *   lya_build_pairs_device  (the same, on the GPU)
  *   lya_build_work      <- per-forest1 launch loop      correlation_procedures_pycuda.py:141-166
  *   lya_xi_run          <- pair_correlation kernel      cuda_kernels.cpp:90-162, launched
  *                          once per forest1 with grid (n1,1,1), block (1,16,16)
@@ -73,6 +74,47 @@ int lya_build_work(int32_t n_f1, const int32_t* f1_list, const int32_t* f1_slot,
                    int64_t item_cost, int64_t tile_cost,
                    int32_t* items, int32_t* tiles, int32_t* n_items, int32_t* n_tiles,
                    double* total_cost);
+
+/*
+ * Tiles from an item list (host): the tile step of lya_build_work applied to
+ * items {f1, nb_begin, nb_end, slot} with their costs (pixel pairs), e.g. the
+ * output of lya_build_items_device.  Call with tiles == NULL for *n_tiles.
+ * Replaces the same part of the per-forest1 launch loop as lya_build_work.
+ */
+int lya_build_tiles(int64_t n_items, const int32_t* items, const int64_t* item_cost,
+                    int64_t tile_cost, int32_t* tiles, int32_t* n_tiles);
+
+/*
+ * Forest-pair list on the GPU: the rule and the output of lya_build_pairs
+ * (identical arrays), for quasar.neighborhood (forest_class.py:78-98) called
+ * for every forest.  ux, uy, uz, ra, name, nb_off (n_forest + 1) and nb_idx
+ * are device pointers; scratch is a device buffer of at least
+ * lya_pairs_device_scratch_bytes(n_forest, cos_max) bytes.
+ * Call with nb_idx == NULL to fill nb_off and get *n_pairs (the call waits for
+ * the stream to read it back); then call again with nb_idx of capacity >=
+ * *n_pairs.  A forest with more than 16384 neighbours returns
+ * LYA_ERR_UNSUPPORTED (use lya_build_pairs).
+ */
+int lya_pairs_device_scratch_bytes(int32_t n_forest, double cos_max, int64_t* bytes);
+int lya_build_pairs_device(int32_t n_forest, const double* ux, const double* uy, const double* uz,
+                           const double* ra, const int64_t* name, double cos_max,
+                           int64_t* nb_off, int32_t* nb_idx, int64_t nb_capacity,
+                           void* scratch, int64_t scratch_bytes, int64_t* n_pairs, void* stream);
+
+/*
+ * Work items on the GPU: the item cut of lya_build_work (same items, same
+ * order) with the cost of each item.  f1_list, f1_slot, pix_off, nb_off,
+ * nb_idx, items (4 * capacity int32) and costs (capacity int64) are device
+ * pointers; scratch is at least lya_items_device_scratch_bytes(n_f1) bytes.
+ * Call with items == NULL for *n_items and *total_cost (waits for the stream);
+ * then with buffers.  Tiles follow from lya_build_tiles on the host copy.
+ */
+int lya_items_device_scratch_bytes(int32_t n_f1, int64_t* bytes);
+int lya_build_items_device(int32_t n_f1, const int32_t* f1_list, const int32_t* f1_slot,
+                           const int64_t* pix_off, const int64_t* nb_off, const int32_t* nb_idx,
+                           int64_t item_cost, int32_t* items, int64_t* costs, int64_t items_capacity,
+                           void* scratch, int64_t scratch_bytes, int64_t* n_items, double* total_cost,
+                           void* stream);
 
 /* One xi problem: all pointers are device pointers except where noted. */
 typedef struct lya_xi_problem {

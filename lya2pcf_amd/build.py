@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "liblya2pcf.so")
 
-HIP_SOURCES = ["xi.hip", "distortion.hip"]
+HIP_SOURCES = ["xi.hip", "distortion.hip", "pairs.hip"]
 HOST_SOURCES = ["host.cpp", "errors.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -106,6 +106,31 @@ extern "C" int lya_build_pairs(int32_t n, const double* ux, const double* uy, co
     return LYA_OK;
 }
 
+namespace {
+
+// Tiles: runs of consecutive items of one slot, cost <= tile_cost (a tile holds at
+// least one item), ordered by decreasing cost (stable).
+void make_tiles(int64_t n_items, const int32_t* items, const int64_t* cost, int64_t tile_cost,
+                std::vector<std::pair<int32_t, int32_t>>& out) {
+    struct Tile { int32_t b, e; int64_t cost; };
+    std::vector<Tile> tl;
+    for (int64_t s = 0; s < n_items;) {
+        int64_t e = s;
+        int64_t acc = 0;
+        while (e < n_items && items[4 * e + 3] == items[4 * s + 3] && (e == s || acc + cost[e] <= tile_cost)) {
+            acc += cost[e];
+            ++e;
+        }
+        tl.push_back({(int32_t)s, (int32_t)e, acc});
+        s = e;
+    }
+    std::stable_sort(tl.begin(), tl.end(), [](const Tile& a, const Tile& b) { return a.cost > b.cost; });
+    out.resize(tl.size());
+    for (size_t k = 0; k < tl.size(); ++k) out[k] = {tl[k].b, tl[k].e};
+}
+
+}  // namespace
+
 extern "C" int lya_build_work(int32_t n_f1, const int32_t* f1_list, const int32_t* f1_slot,
                               const int64_t* pix_off, const int64_t* nb_off, const int32_t* nb_idx,
                               int64_t item_cost, int64_t tile_cost, int32_t* items, int32_t* tiles,
@@ -117,57 +142,81 @@ extern "C" int lya_build_work(int32_t n_f1, const int32_t* f1_list, const int32_
         if (f1_slot[k] < f1_slot[k - 1])
             return lya_fail(LYA_ERR_ARG, "lya_build_work: f1_list must be grouped by ascending slot");
 
-    struct Item { int32_t f1, b, e, slot; int64_t cost; };
-    std::vector<Item> it;
-    double total = 0.0;
-    for (int32_t k = 0; k < n_f1; ++k) {
-        int32_t f = f1_list[k];
-        int64_t n1 = pix_off[f + 1] - pix_off[f];
-        int64_t b = nb_off[f], e = nb_off[f + 1];
-        int64_t s = b, acc = 0;
+    // Items: the neighbours of each forest 1 cut greedily into runs of cost <=
+    // item_cost (at least one neighbour each).  Forests are independent, so the
+    // cut runs in parallel: count, prefix sum, fill.
+    auto cut = [&](int32_t k, auto&& emit) {
+        const int32_t f = f1_list[k];
+        const int64_t n1 = pix_off[f + 1] - pix_off[f];
+        const int64_t b = nb_off[f], e = nb_off[f + 1];
+        int64_t s = b, acc = 0, tot = 0;
         for (int64_t q = b; q < e; ++q) {
-            int32_t f2 = nb_idx[q];
-            int64_t c = n1 * (pix_off[f2 + 1] - pix_off[f2]);
-            total += (double)c;
+            const int32_t f2 = nb_idx[q];
+            const int64_t c = n1 * (pix_off[f2 + 1] - pix_off[f2]);
+            tot += c;
             if (q > s && acc + c > item_cost) {
-                it.push_back({f, (int32_t)s, (int32_t)q, f1_slot[k], acc});
+                emit(f, s, q, acc);
                 s = q;
                 acc = 0;
             }
             acc += c;
         }
-        if (e > s) it.push_back({f, (int32_t)s, (int32_t)e, f1_slot[k], acc});
+        if (e > s) emit(f, s, e, acc);
+        return tot;
+    };
+    std::vector<int64_t> first(n_f1 + 1, 0);
+    double total = 0.0;  // integer costs, exact in double below 2^53: any summation order
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : total)
+    for (int32_t k = 0; k < n_f1; ++k) {
+        int64_t ni = 0;
+        total += (double)cut(k, [&](int32_t, int64_t, int64_t, int64_t) { ++ni; });
+        first[k + 1] = ni;
     }
-    if (it.size() > (size_t)INT32_MAX) return lya_fail(LYA_ERR_CAPACITY, "lya_build_work: too many items");
-
-    // tiles: runs of items of one slot, cost <= tile_cost
-    struct Tile { int32_t b, e; int64_t cost; };
-    std::vector<Tile> tl;
-    for (size_t s = 0; s < it.size();) {
-        size_t e = s;
-        int64_t acc = 0;
-        while (e < it.size() && it[e].slot == it[s].slot && (e == s || acc + it[e].cost <= tile_cost)) {
-            acc += it[e].cost;
-            ++e;
-        }
-        tl.push_back({(int32_t)s, (int32_t)e, acc});
-        s = e;
+    for (int32_t k = 0; k < n_f1; ++k) first[k + 1] += first[k];
+    const int64_t ni = first[n_f1];
+    if (ni > (int64_t)INT32_MAX) return lya_fail(LYA_ERR_CAPACITY, "lya_build_work: too many items");
+    std::vector<int32_t> it(4 * (size_t)ni);
+    std::vector<int64_t> cost(ni);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int32_t k = 0; k < n_f1; ++k) {
+        int64_t o = first[k];
+        cut(k, [&](int32_t f, int64_t s, int64_t e, int64_t acc) {
+            it[4 * o + 0] = f;
+            it[4 * o + 1] = (int32_t)s;
+            it[4 * o + 2] = (int32_t)e;
+            it[4 * o + 3] = f1_slot[k];
+            cost[o] = acc;
+            ++o;
+        });
     }
-    std::stable_sort(tl.begin(), tl.end(), [](const Tile& a, const Tile& b) { return a.cost > b.cost; });
+    std::vector<std::pair<int32_t, int32_t>> tl;
+    make_tiles(ni, it.data(), cost.data(), tile_cost, tl);
 
-    *n_items_out = (int32_t)it.size();
+    *n_items_out = (int32_t)ni;
     *n_tiles_out = (int32_t)tl.size();
     if (total_cost) *total_cost = total;
     if (!items || !tiles) return LYA_OK;
-    for (size_t k = 0; k < it.size(); ++k) {
-        items[4 * k + 0] = it[k].f1;
-        items[4 * k + 1] = it[k].b;
-        items[4 * k + 2] = it[k].e;
-        items[4 * k + 3] = it[k].slot;
-    }
+    std::copy(it.begin(), it.end(), items);
     for (size_t k = 0; k < tl.size(); ++k) {
-        tiles[2 * k + 0] = tl[k].b;
-        tiles[2 * k + 1] = tl[k].e;
+        tiles[2 * k + 0] = tl[k].first;
+        tiles[2 * k + 1] = tl[k].second;
+    }
+    return LYA_OK;
+}
+
+extern "C" int lya_build_tiles(int64_t n_items, const int32_t* items, const int64_t* item_cost,
+                               int64_t tile_cost, int32_t* tiles, int32_t* n_tiles) {
+    if (n_items < 0 || !n_tiles || (n_items > 0 && (!items || !item_cost)))
+        return lya_fail(LYA_ERR_ARG, "lya_build_tiles: null argument or n_items < 0");
+    if (tile_cost < 1) return lya_fail(LYA_ERR_ARG, "lya_build_tiles: tile_cost must be >= 1");
+    if (n_items > (int64_t)INT32_MAX) return lya_fail(LYA_ERR_CAPACITY, "lya_build_tiles: too many items");
+    std::vector<std::pair<int32_t, int32_t>> tl;
+    make_tiles(n_items, items, item_cost, tile_cost, tl);
+    *n_tiles = (int32_t)tl.size();
+    if (!tiles) return LYA_OK;
+    for (size_t k = 0; k < tl.size(); ++k) {
+        tiles[2 * k + 0] = tl[k].first;
+        tiles[2 * k + 1] = tl[k].second;
     }
     return LYA_OK;
 }

@@ -11,6 +11,8 @@ with the binning of reference cuda_kernels.cpp:90-162 and the pair list of
 forest_class.py:78-98.  Device buffers are torch tensors (plumbing only).
 """
 import ctypes as C
+import time
+
 import numpy as np
 
 from . import _lib
@@ -63,6 +65,79 @@ def build_work(forests, nb_off, nb_idx, f1_list, item_cost, tile_cost):
     return items[:ni.value], tiles[:nt.value], tot.value
 
 
+def build_tiles(items, costs, tile_cost):
+    """Tiles {item_begin, item_end} (host int32 [n_tiles, 2]) from items and their costs."""
+    L = lib()
+    items = np.ascontiguousarray(items, dtype=np.int32).reshape(-1, 4)
+    costs = np.ascontiguousarray(costs, dtype=np.int64)
+    if len(costs) != len(items):
+        raise ValueError("build_tiles: %d items but %d costs" % (len(items), len(costs)))
+    nt = C.c_int32(0)
+    check(L.lya_build_tiles(len(items), ptr(items), ptr(costs), int(tile_cost), None, C.byref(nt)))
+    tiles = np.zeros((max(1, nt.value), 2), dtype=np.int32)
+    check(L.lya_build_tiles(len(items), ptr(items), ptr(costs), int(tile_cost), ptr(tiles), C.byref(nt)))
+    return tiles[:nt.value]
+
+
+def _device_vec(t, dtype, n, what):
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == dtype and t.is_contiguous() and t.numel() == n):
+        raise ValueError("%s must be a contiguous %s device tensor of %d elements" % (what, dtype, n))
+    return t
+
+
+def build_pairs_device(ux, uy, uz, ra, name, angmax, stream=None):
+    """The neighbour list of :func:`build_pairs`, built on the GPU from device tensors
+    (ux, uy, uz, ra float64; name int64).  Returns (nb_off int64 [nf+1], nb_idx int32
+    [n_pairs]) device tensors and n_pairs; the arrays equal the host builder's."""
+    import torch
+    L = lib()
+    n = ux.numel()
+    for t, what in ((ux, "ux"), (uy, "uy"), (uz, "uz"), (ra, "ra")):
+        _device_vec(t, torch.float64, n, what)
+    _device_vec(name, torch.int64, n, "name")
+    cosm = float(np.cos(angmax))
+    nbytes = C.c_int64(0)
+    check(L.lya_pairs_device_scratch_bytes(n, cosm, C.byref(nbytes)))
+    dev = ux.device
+    scratch = torch.empty(max(1, nbytes.value), dtype=torch.uint8, device=dev)
+    nb_off = torch.empty(n + 1, dtype=torch.int64, device=dev)
+    npairs = C.c_int64(0)
+    sp = _lib.stream_ptr(stream)
+    args = (n, ptr(ux), ptr(uy), ptr(uz), ptr(ra), ptr(name), cosm, ptr(nb_off))
+    check(L.lya_build_pairs_device(*args, None, 0, ptr(scratch), nbytes.value, C.byref(npairs), sp))
+    nb_idx = torch.empty(max(1, npairs.value), dtype=torch.int32, device=dev)
+    check(L.lya_build_pairs_device(*args, ptr(nb_idx), nb_idx.numel(), ptr(scratch), nbytes.value, C.byref(npairs),
+                                   sp))
+    return nb_off, nb_idx[:npairs.value], npairs.value
+
+
+def build_items_device(f1_list, f1_slot, pix_off, nb_off, nb_idx, item_cost, stream=None):
+    """The item cut of :func:`build_work` on the GPU.  Device tensors in (f1_list,
+    f1_slot int32; pix_off, nb_off int64; nb_idx int32); returns (items int32 [n, 4],
+    costs int64 [n]) device tensors and the total pixel-pair count."""
+    import torch
+    L = lib()
+    n = f1_list.numel()
+    _device_vec(f1_list, torch.int32, n, "f1_list")
+    _device_vec(f1_slot, torch.int32, n, "f1_slot")
+    nbytes = C.c_int64(0)
+    check(L.lya_items_device_scratch_bytes(n, C.byref(nbytes)))
+    dev = f1_list.device
+    scratch = torch.empty(max(1, nbytes.value), dtype=torch.uint8, device=dev)
+    ni = C.c_int64(0)
+    tot = C.c_double(0.0)
+    sp = _lib.stream_ptr(stream)
+    nbi = nb_idx if nb_idx.numel() else torch.zeros(1, dtype=torch.int32, device=dev)
+    args = (n, ptr(f1_list), ptr(f1_slot), ptr(pix_off), ptr(nb_off), ptr(nbi), int(item_cost))
+    check(L.lya_build_items_device(*args, None, None, 0, ptr(scratch), nbytes.value, C.byref(ni), C.byref(tot), sp))
+    items = torch.empty((max(1, ni.value), 4), dtype=torch.int32, device=dev)
+    costs = torch.empty(max(1, ni.value), dtype=torch.int64, device=dev)
+    check(L.lya_build_items_device(*args, ptr(items), ptr(costs), ni.value, ptr(scratch), nbytes.value, C.byref(ni),
+                                   C.byref(tot), sp))
+    return items[:ni.value], costs[:ni.value], tot.value
+
+
 def default_config(mode="mixed", **kw):
     cfg = XiConfig()
     cfg.mode = MODES[mode] if isinstance(mode, str) else int(mode)
@@ -85,18 +160,19 @@ class XiEngine:
     """
 
     def __init__(self, forests, angmax, np_rp=50, np_rt=50, rpmax=200.0, rtmax=200.0, slots=None,
-                 item_cost=2_000_000, tiles_per_group=6, device="cuda", pairs=None, transposed=False):
+                 item_cost=2_000_000, tiles_per_group=6, device="cuda", pairs=None, transposed=False,
+                 pair_builder="host"):
         import torch
+        if pair_builder not in ("host", "device"):
+            raise ValueError("pair_builder must be 'host' or 'device'")
         self.torch = torch
         self.forests = forests
         self.angmax = float(angmax)
         self.np_rp, self.np_rt = int(np_rp), int(np_rt)
         self.rpmax, self.rtmax = float(rpmax), float(rtmax)
         self.device = torch.device(device)
-        nb_off, nb_idx = pairs if pairs is not None else build_pairs(forests, angmax)
-        if transposed:
-            nb_off, nb_idx = transpose_pairs(nb_off, nb_idx, forests.nf)
-        self.nb_off, self.nb_idx = nb_off, nb_idx
+        # a given or transposed list is a host list
+        self.pair_builder = pair_builder if (pairs is None and not transposed) else "host"
         self.n_slots = len(forests.slots)
         if slots is None:
             f1_list = np.arange(forests.nf, dtype=np.int32)
@@ -108,10 +184,53 @@ class XiEngine:
         self.item_cost = int(item_cost)
         self.tiles_per_group = tiles_per_group
         self._work = {}
-        # total pixel pairs over the list (the metric's numerator)
-        _, _, self.pixel_pairs = build_work(forests, nb_off, nb_idx, f1_list, self.item_cost, 1 << 62)
-        self._upload()
+        self._items_host = None
+        self.setup_seconds = {}
+        clock = time.perf_counter
+        if self.pair_builder == "host":
+            t0 = clock()
+            nb_off, nb_idx = pairs if pairs is not None else build_pairs(forests, angmax)
+            if transposed:
+                nb_off, nb_idx = transpose_pairs(nb_off, nb_idx, forests.nf)
+            self._nb_host = (nb_off, nb_idx)
+            self.n_pairs = len(nb_idx)
+            t1 = clock()
+            # total pixel pairs over the list (the metric's numerator)
+            _, _, self.pixel_pairs = build_work(forests, nb_off, nb_idx, f1_list, self.item_cost, 1 << 62)
+            self.setup_seconds = {"pairs": t1 - t0, "items": clock() - t1}
+            self._upload()
+        else:
+            self._nb_host = None
+            self._upload()
+            d = self.d
+            torch.cuda.synchronize()
+            t0 = clock()
+            self.d_nb_off, nb_idx_t, self.n_pairs = build_pairs_device(d["ux"], d["uy"], d["uz"], d["ra"], d["name"],
+                                                                       angmax)
+            d["nb_idx"] = nb_idx_t if self.n_pairs else torch.zeros(1, dtype=torch.int32, device=self.device)
+            torch.cuda.synchronize()
+            t1 = clock()
+            f1_slot = np.ascontiguousarray(forests.slot[f1_list], dtype=np.int32)
+            items, costs, self.pixel_pairs = build_items_device(self._t(f1_list), self._t(f1_slot), d["pix_off"],
+                                                                self.d_nb_off, nb_idx_t, self.item_cost)
+            self._items_host = (items.cpu().numpy(), costs.cpu().numpy())
+            self.setup_seconds = {"pairs": t1 - t0, "items": clock() - t1}
         self._graphs = []
+
+    def _host_pairs(self):
+        if self._nb_host is None:
+            self._nb_host = (self.d_nb_off.cpu().numpy(), self.d["nb_idx"][:self.n_pairs].cpu().numpy())
+        return self._nb_host
+
+    @property
+    def nb_off(self):
+        """CSR offsets of the forest-pair list (host copy)."""
+        return self._host_pairs()[0]
+
+    @property
+    def nb_idx(self):
+        """Neighbour forest indices of the forest-pair list (host copy)."""
+        return self._host_pairs()[1]
 
     # ------------------------------------------------------------------
     def _t(self, a, dtype=None):
@@ -128,8 +247,12 @@ class XiEngine:
             "base": self._t(f.base), "sorted": self._t(f.sorted),
             "dc": self._t(f.dc), "we": self._t(f.we), "dw": self._t(f.dw),
             "off32": self._t(f.off32), "we32": self._t(f.we32), "dw32": self._t(f.dw32),
-            "nb_idx": self._t(self.nb_idx if len(self.nb_idx) else np.zeros(1, np.int32)),
         }
+        if self.pair_builder == "host":
+            self.d["nb_idx"] = self._t(self.nb_idx if len(self.nb_idx) else np.zeros(1, np.int32))
+        else:
+            self.d["ra"] = self._t(np.ascontiguousarray(f.ra, dtype=np.float64))
+            self.d["name"] = self._t(np.ascontiguousarray(f.name, dtype=np.int64))
         nbins = self.np_rp * self.np_rt
         self.out_cnt = torch.zeros((self.n_slots, nbins), dtype=torch.int64, device=self.device)
         self.out_w = torch.zeros((self.n_slots, nbins), dtype=torch.float64, device=self.device)
@@ -155,8 +278,12 @@ class XiEngine:
         key = groups.value
         if key not in self._work:
             tile_cost = max(self.item_cost, int(self.pixel_pairs / max(1, key * self.tiles_per_group)))
-            items, tiles, _ = build_work(self.forests, self.nb_off, self.nb_idx, self.f1_list, self.item_cost,
-                                         tile_cost)
+            if self._items_host is not None:
+                items, costs = self._items_host
+                tiles = build_tiles(items, costs, tile_cost)
+            else:
+                items, tiles, _ = build_work(self.forests, self.nb_off, self.nb_idx, self.f1_list, self.item_cost,
+                                             tile_cost)
             self._work[key] = (self._t(items if len(items) else np.zeros((1, 4), np.int32)),
                                self._t(tiles if len(tiles) else np.zeros((1, 2), np.int32)),
                                len(items), len(tiles))
