@@ -69,7 +69,8 @@ def main(argv=None):
         correlations.init(data, log, shape_hist, angmax)
     else:
         import correlation_procedures_hip as correlations
-        correlations.init(data, log, shape_hist, angmax, compute_pixels=list(mine), precision=args.precision)
+        correlations.init(data, log, shape_hist, angmax, compute_pixels=list(mine), precision=args.precision,
+                          counts=args.counts)
 
     for k, pixel in enumerate(mine):
         log.write("\nComputing pixel %s, completed %d%%" % (pixel, int(k / max(1, len(mine)) * 100)))

@@ -187,11 +187,25 @@ def main():
     angmax = forests.angmax_from(fs, 200.0)
     t_gen = time.perf_counter() - t0
     t0 = time.perf_counter()
-    eng = xi.XiEngine(fs, angmax)
+    eng = xi.XiEngine(fs, angmax, pair_builder="device")
     t_setup = time.perf_counter() - t0
     P = eng.pixel_pairs
-    log("data: %d forests, %d pixels, %d forest pairs, P=%.4g pixel pairs (gen %.1fs, pairs+upload %.1fs)"
-        % (fs.nf, fs.n_pix, len(eng.nb_idx), P, t_gen, t_setup))
+    log("data: %d forests, %d pixels, %d forest pairs, P=%.4g pixel pairs (gen %.1fs, upload+pairs %.1fs)"
+        % (fs.nf, fs.n_pix, eng.n_pairs, P, t_gen, t_setup))
+    # set-up outside the timed region: pair list and work items on the GPU, and the
+    # host builders (OpenMP) for comparison
+    setup = {"pair_list_gpu_s": eng.setup_seconds["pairs"], "work_items_gpu_s": eng.setup_seconds["items"]}
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        off_h, idx_h = xi.build_pairs(fs, angmax)
+        setup["pair_list_host_s"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        xi.build_work(fs, off_h, idx_h, eng.f1_list, eng.item_cost, 1 << 62)
+        setup["work_items_host_s"] = time.perf_counter() - t0
+        setup["host_threads"] = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))
+        setup["pair_lists_identical"] = bool(np.array_equal(off_h, eng.nb_off) and np.array_equal(idx_h, eng.nb_idx))
+        del off_h, idx_h
+    log("setup: %s" % setup)
 
     def shape_for(mode):
         best = autotune.load_best(mode)
@@ -236,13 +250,14 @@ def main():
         "metric": METRIC, "value": value, "unit": "pixel-pairs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": m["wall"] * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64" if main_mode == "fp64" else "f32+f64", "data": "synthetic",
-        "config": {"workload": cfg.name, "forests": fs.nf, "pixels": fs.n_pix, "forest_pairs": int(len(eng.nb_idx)),
+        "config": {"workload": cfg.name, "forests": fs.nf, "pixels": fs.n_pix, "forest_pairs": int(eng.n_pairs),
                    "pixel_pairs": P, "bins": "50x50 of 4 Mpc/h", "kernel": main_mode, "shape": out[main_mode]["shape"],
                    "parallelism": "single GPU, persistent work queue"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": autotune.measured_traffic(main_mode),
                      "note": "10 FLOP per pixel pair (algorithmic) / median per-launch hipEvent time; "
                              "no MFMA or HBM bound applies"},
+        "setup": setup,
     }
     if "fp64" in out and "mixed" in out:
         c64, w64, d64 = out["fp64"]["res"]

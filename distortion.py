@@ -28,7 +28,6 @@ def main(argv=None):
     size = comm.Get_size() if comm else 1
     os.environ.setdefault("HIP_VISIBLE_DEVICES", str(rank % P.number_of_cuda_devices + P.cuda_device_first_number))
     import distortion_procedures_hip as distortion
-    from lya2pcf_amd.distortion import subsample
 
     ap = argparse.ArgumentParser(description="Distortion matrix of the two-point correlation function.")
     ap.add_argument("--excluded", default=0.95, help="fraction of forest pairs excluded")
@@ -49,17 +48,7 @@ def main(argv=None):
 
     # device-side accumulation over the rank's pixels (same subsample sequence as
     # calling distortion_per_pixel pixel by pixel)
-    from lya2pcf_amd.distortion import _S
-    eng = _S["eng"]
-    eng.zero()
-    for k, pixel in enumerate(mine):
-        log.write("\nComputing pixel %s, completed %d%%" % (pixel, int(k / max(1, len(mine)) * 100)))
-        log.flush()
-        f1 = [_S["index"][id(q)] for q in data[pixel]]
-        eng.accumulate(subsample(_S["fs"], eng.nb_off, eng.nb_idx, f1, _S["reject"]))
-        if args.verbose and k > 0:
-            break
-    D, wA = eng.result()
+    D, wA = distortion.distortion_of_pixels(mine[:2] if args.verbose else mine, log)
     if comm is not None and size > 1:
         D = comm.reduce(D)
         wA = comm.reduce(wA)

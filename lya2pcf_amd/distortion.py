@@ -54,10 +54,16 @@ class DistortionEngine:
         self.np_rp, self.np_rt = int(np_rp), int(np_rt)
         self.rpmax, self.rtmax = float(rpmax), float(rtmax)
         self.nb = self.np_rp * self.np_rt
-        self.nb_off, self.nb_idx = pairs if pairs is not None else _xi.build_pairs(fs, angmax)
         dev = torch.device("cuda")
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
         self.d = {k: t(getattr(fs, k)) for k in ("pix_off", "dc", "we", "dl", "ux", "uy", "uz", "omega", "odl2")}
+        if pairs is None:
+            # the pair list is built on the GPU; the subsampling walks a host copy
+            off, idx, _ = _xi.build_pairs_device(self.d["ux"], self.d["uy"], self.d["uz"],
+                                                 t(np.asarray(fs.ra, dtype=np.float64)),
+                                                 t(np.asarray(fs.name, dtype=np.int64)), angmax)
+            pairs = (off.cpu().numpy(), idx.cpu().numpy())
+        self.nb_off, self.nb_idx = pairs
         self.max_len = int(np.max(fs.n)) if fs.nf else 1
         props = torch.cuda.get_device_properties(0)
         self.groups = groups or props.multi_processor_count
@@ -104,6 +110,7 @@ def init(data_aux, log_file_aux, shape_hist_aux, angmax_aux, reject_aux, pixel_l
     fs = _forests.ForestSet.from_data(data_aux, pixel_list)
     _S.clear()
     _S["fs"] = fs
+    _S["data"] = data_aux
     _S["index"] = {id(q): k for k, q in enumerate(fs.objects)}
     _S["reject"] = float(reject_aux)
     np_rp, np_rt = (int(x) for x in shape_hist_aux)
@@ -116,4 +123,20 @@ def distortion_per_pixel(forest_list, **kargs):
     pairs = subsample(_S["fs"], eng.nb_off, eng.nb_idx, f1, _S["reject"])
     eng.zero()
     eng.accumulate(pairs)
+    return eng.result()
+
+
+def distortion_of_pixels(pixels, log_file=None):
+    """Sum of ``distortion_per_pixel`` over the forests of ``pixels`` (in that
+    order, so the subsample draws the same random sequence), accumulated on the
+    device in FP64: (D f64 [nb, nb], weight f64 [nb])."""
+    eng = _S["eng"]
+    data = _S["data"]
+    eng.zero()
+    for k, pixel in enumerate(pixels):
+        if log_file is not None:
+            log_file.write("\nComputing pixel %s, completed %d%%" % (pixel, int(k / max(1, len(pixels)) * 100)))
+            log_file.flush()
+        f1 = [_S["index"][id(q)] for q in data[pixel]]
+        eng.accumulate(subsample(_S["fs"], eng.nb_off, eng.nb_idx, f1, _S["reject"]))
     return eng.result()

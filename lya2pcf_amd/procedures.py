@@ -14,6 +14,9 @@ the captured persistent kernel; later calls return the cached histograms.
 Extra keyword arguments of ``init`` (all optional):
   compute_pixels  pixels that will be asked for (default: every pixel of data)
   precision       'fp64' (reference formula, default) or 'mixed'
+  config          an XiConfig (default: the tuned shape of the precision)
+  counts          also count pixel pairs per bin (default True; off saves one
+                  LDS atomic per pair)
   rpmax, rtmax    default from the ``parameters`` module
 Pair counts per bin are available from ``pair_counts(pixel)``.
 """
@@ -34,7 +37,7 @@ def _params():
 
 
 def init(data_aux, log_file_aux, shape_hist_aux, angmax_aux, pixel_list=None, compute_pixels=None,
-         precision="fp64", rpmax=None, rtmax=None, config=None):
+         precision="fp64", rpmax=None, rtmax=None, config=None, counts=True):
     dr, dt = _params()
     _S.clear()
     _S["log"] = log_file_aux
@@ -48,10 +51,10 @@ def init(data_aux, log_file_aux, shape_hist_aux, angmax_aux, pixel_list=None, co
               engine=_xi.XiEngine(fs, float(angmax_aux), np_rp=np_rp, np_rt=np_rt,
                                   rpmax=rpmax if rpmax is not None else dr,
                                   rtmax=rtmax if rtmax is not None else dt, slots=want),
-              config=config, done=False)
+              config=config, counts=bool(counts), done=False)
     if log_file_aux is not None:
         log_file_aux.write("\nHIP pair list: %d forests, %d forest pairs, %.4g pixel pairs"
-                           % (fs.nf, len(_S["engine"].nb_idx), _S["engine"].pixel_pairs))
+                           % (fs.nf, _S["engine"].n_pairs, _S["engine"].pixel_pairs))
         log_file_aux.flush()
 
 
@@ -59,7 +62,14 @@ def _compute():
     if _S.get("done"):
         return
     eng = _S["engine"]
-    cfg = _S["config"] or _xi.default_config(_S["precision"])
+    cfg = _S["config"]
+    if cfg is None:
+        # the tuned kernel shape of this precision, if a tuning table is present
+        from . import autotune
+        shape = autotune.load_best(_S["precision"]) or {}
+        cfg = _xi.default_config(_S["precision"], counts=_S["counts"],
+                                 **{k: v for k, v in shape.items()
+                                    if k in ("px_per_lane", "waves", "replicas", "row_pad", "lane_map")})
     g = eng.graph(cfg)
     g.launch()
     _S["cnt"], _S["w"], _S["dw"] = eng.results()
@@ -81,6 +91,8 @@ def two_point_per_pixel(pixel, **kargs):
 
 def pair_counts(pixel):
     """Number of pixel pairs per bin (uint64) for HEALPix ``pixel``."""
+    if not _S.get("counts", True):
+        raise RuntimeError("pair counts were not requested (init(..., counts=False))")
     _compute()
     k = _S["slot_of"].get(pixel)
     if k is None:

@@ -157,11 +157,14 @@ class XiEngine:
     forests   ForestSet (host); uploaded once.
     angmax    neighbour cut (radians).
     slots     slot indices whose forests act as forest 1 (default: all).
+    pair_builder  "device" (default): the forest-pair list and the work items are
+              built on the GPU and stay resident; "host": OpenMP builders.  Both
+              give identical arrays.  A given (pairs=) or transposed list is host.
     """
 
     def __init__(self, forests, angmax, np_rp=50, np_rt=50, rpmax=200.0, rtmax=200.0, slots=None,
                  item_cost=2_000_000, tiles_per_group=6, device="cuda", pairs=None, transposed=False,
-                 pair_builder="host"):
+                 pair_builder="device"):
         import torch
         if pair_builder not in ("host", "device"):
             raise ValueError("pair_builder must be 'host' or 'device'")

@@ -117,3 +117,78 @@ def test_distortion_driver(tmp_path):
     assert np.array_equal(np.isfinite(D).all(1), ok)
     scale = np.abs(E[ok]).max()
     assert np.max(np.abs(D[ok] - E[ok])) <= 1e-10 * scale
+
+
+# ------------------------------------------------------------ multi-file drivers
+def _two_files(tmp_path):
+    """data1.npy and data2.npy: two surveys over the same patch (shared pixels)."""
+    s1 = _workspace(tmp_path, n=80, seed=21, length_range=(30, 80))
+    s2 = synth.generate(synth.scaled(synth.CONFIGS["c0"], 70, seed=22, length_range=(30, 80)))
+    synth.write_data_npy(s2, str(tmp_path / "deltas" / "data2.npy"))
+    for f in ("2pla_multiple_data.py", "distortion_multiple_data.py"):
+        shutil.copy(os.path.join(ROOT, f), tmp_path / f)
+    return [s1, s2]
+
+
+def test_driver_helpers(tmp_path):
+    from lya2pcf_amd import drivers
+    for name in ("data2.npy", "data10.npy", "data1.npy", "other.npy"):
+        (tmp_path / name).write_bytes(b"")
+    files = drivers.data_files(str(tmp_path))
+    assert [os.path.basename(f) for f in files] == ["data1.npy", "data10.npy", "data2.npy"]
+    parts = [drivers.files_of_rank(str(tmp_path), r, 2) for r in range(2)]
+    assert parts[0] + parts[1] == files and len(parts[0]) == 2
+    with pytest.raises(FileNotFoundError):
+        drivers.data_files(str(tmp_path / "nothing"))
+    s = synth.generate(synth.scaled(synth.CONFIGS["c0"], 30, seed=4))
+    dmin, am = drivers.cut_angle(synth.to_data_dict(s), 200.0)
+    fs = forests.ForestSet.from_survey(s)
+    assert dmin == fs.dc[fs.pix_off[:-1]].min() and am == forests.angmax_from(fs, 200.0)
+    with pytest.raises(NameError):
+        drivers.cut_angle({}, 200.0)
+
+
+@pytest.mark.gpu
+def test_2pla_multiple_data_outputs(tmp_path):
+    surveys = _two_files(tmp_path)
+    _run(tmp_path, "2pla_multiple_data.py", "--gpu", "--counts")
+    out = tmp_path / "out"
+    expect = {}
+    for s in surveys:  # sorted file order: the later file wins a shared pixel
+        fs = forests.ForestSet.from_survey(s)
+        off, idx = oracle.pairs(fs, forests.angmax_from(fs, 200.0))  # pairs within the file only
+        c, w, d = oracle.xi_gpu_form(fs, off, idx)
+        for k, pix in enumerate(fs.slots):
+            expect[pix] = (c[k], w[k], d[k])
+    shared = set(forests.ForestSet.from_survey(surveys[0]).slots) & set(forests.ForestSet.from_survey(surveys[1]).slots)
+    assert shared, "the two files should share pixels"
+    files = sorted(out.glob("2d_histogram_pixel_*"))
+    assert len(files) == len(expect)
+    for pix, (c0, w0, d0) in expect.items():
+        w_hist, dw_hist = np.load(out / ("2d_histogram_pixel_%d.npy" % pix))
+        assert np.array_equal(np.load(out / ("2d_paircounts_pixel_%d.npy" % pix)), c0)
+        np.testing.assert_allclose(w_hist, w0, rtol=1e-12)
+        np.testing.assert_allclose(dw_hist, d0, rtol=1e-9, atol=1e-9 * np.abs(w0).max())
+
+
+@pytest.mark.gpu
+def test_distortion_multiple_data(tmp_path):
+    import random
+    from lya2pcf_amd import distortion as dist
+    surveys = _two_files(tmp_path)
+    _run(tmp_path, "distortion_multiple_data.py", "--excluded", "0.8")
+    D = np.load(tmp_path / "out" / "distortion.npy")
+    rng = random.Random(1)  # one stream, continued from file to file
+    D0 = np.zeros((2500, 2500))
+    w0 = np.zeros(2500)
+    for s in surveys:
+        fs = forests.ForestSet.from_survey(s)
+        off, idx = oracle.pairs(fs, forests.angmax_from(fs, 200.0))
+        d, w = oracle.distortion(fs, dist.subsample(fs, off, idx, range(fs.nf), 0.8, rng=rng))
+        D0 += d
+        w0 += w
+    with np.errstate(divide="ignore", invalid="ignore"):
+        E = D0 / w0[:, None]
+    ok = w0 > 0
+    assert np.array_equal(np.isfinite(D).all(1), ok)
+    assert np.max(np.abs(D[ok] - E[ok])) <= 1e-10 * np.abs(E[ok]).max()

@@ -161,7 +161,7 @@ def test_engine_device_builder_same_results(small_set, mode):
     fs = small_set
     am = forests.angmax_from(fs, 200.0)
     print("engine device builder", mode, flush=True)
-    e_h = xi.XiEngine(fs, am, item_cost=40_000)
+    e_h = xi.XiEngine(fs, am, item_cost=40_000, pair_builder="host")
     e_d = xi.XiEngine(fs, am, item_cost=40_000, pair_builder="device")
     assert e_d.pixel_pairs == e_h.pixel_pairs and e_d.n_pairs == e_h.n_pairs
     assert np.array_equal(e_d.nb_off, e_h.nb_off) and np.array_equal(e_d.nb_idx, e_h.nb_idx)
