@@ -143,6 +143,10 @@ typedef struct lya_xi_problem {
     double rtmax;
     float tau;                /* mixed: half-width of the FP64 recheck band, bin units */
     int32_t lane_map;         /* lane -> pixel order: 0 consecutive, 1 by 2, 2 by 4   */
+    int32_t max_len;          /* host: longest forest in pixels (walk 1: <= 16000)    */
+    int32_t pad;              /* dc, we, dw readable at indices [-pad, 0) and         */
+                              /* [n_pix, n_pix + pad); walk 1 needs pad >= max_len,  */
+                              /* dc = 1e300 and we, dw finite there                  */
     uint64_t* out_cnt;        /* [n_slots][np_rp*np_rt] pair counts                   */
     double* out_w;            /* [n_slots][np_rp*np_rt] sum w1*w2                     */
     double* out_dw;           /* [n_slots][np_rp*np_rt] sum dw1*dw2                   */
@@ -158,6 +162,9 @@ typedef struct lya_xi_config {
     int32_t row_pad;          /* extra LDS columns per r_parallel row: 0 or 1         */
     int32_t groups;           /* workgroups in the persistent grid (0 = fill the GPU)  */
     int32_t no_counts;        /* 1: skip the per-bin pair counts (out_cnt left zero)  */
+    int32_t walk;             /* 0: lanes hold forest-1 pixels (all modes);           */
+                              /* 1: lanes walk diagonals (i, i+d) of each forest pair */
+                              /*    and sum runs of equal bins in registers (FP64)    */
 } lya_xi_config;
 
 /* Largest mixed-mode error bound for these data (host): returns tau in bin units. */

@@ -29,7 +29,8 @@ class XiProblem(C.Structure):
         ("dc", _vp), ("we", _vp), ("dw", _vp), ("off32", _vp), ("we32", _vp), ("dw32", _vp),
         ("nb_idx", _vp), ("items", _vp), ("tiles", _vp),
         ("n_items", _i32), ("n_tiles", _i32), ("np_rp", _i32), ("np_rt", _i32),
-        ("rpmax", _f64), ("rtmax", _f64), ("tau", C.c_float), ("lane_map", _i32),
+        ("rpmax", _f64), ("rtmax", _f64), ("tau", C.c_float), ("lane_map", _i32), ("max_len", _i32),
+        ("pad", _i32),
         ("out_cnt", _vp), ("out_w", _vp), ("out_dw", _vp), ("queue", _vp),
     ]
 
@@ -38,7 +39,7 @@ class XiConfig(C.Structure):
     """Mirror of ``lya_xi_config`` (``lane_map`` is a Python-side field copied into
     the problem's ``lane_map``)."""
     _fields_ = [("mode", _i32), ("px_per_lane", _i32), ("waves", _i32), ("replicas", _i32),
-                ("row_pad", _i32), ("groups", _i32), ("no_counts", _i32)]
+                ("row_pad", _i32), ("groups", _i32), ("no_counts", _i32), ("walk", _i32)]
     lane_map = 1
 
     def as_dict(self):

@@ -34,6 +34,7 @@
 //                   (fewer LDS bytes, ~1e-6 relative error on the sums).
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <type_traits>
@@ -90,6 +91,30 @@ __device__ __forceinline__ void halfangles(double x1, double y1, double z1, doub
 __device__ __forceinline__ uint32_t opaque(uint32_t x) {
     asm volatile("" : "+v"(x));
     return x;
+}
+
+// Forest-2 pixels are wave-uniform.  With LYA_VMEM_J (default) they are read
+// with vector loads at a uniform address (the index is made opaque so the
+// compiler cannot turn them into scalar loads): vector loads complete on
+// vmcnt, while scalar loads share lgkmcnt with the LDS atomics, so every use of
+// a scalar-loaded value behind a skipped (exec-masked) accumulate would wait
+// for all atomics issued before it.  With vector loads the atomics are never
+// waited for inside the pair loop.
+#ifndef LYA_VMEM_J
+#define LYA_VMEM_J 0
+#endif
+template <class T>
+__device__ __forceinline__ auto stream(const T* base) {
+    if constexpr (LYA_VMEM_J)
+        return base;
+    else
+        return cptr(base);
+}
+__device__ __forceinline__ int jidx(int j) {
+    if constexpr (LYA_VMEM_J)
+        return (int)opaque((uint32_t)j);
+    else
+        return j;
 }
 
 template <class T>
@@ -193,9 +218,9 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
             w1[k] = valid[k] ? p.we[pix1[k]] : 0.0;
             d1[k] = valid[k] ? p.dw[pix1[k]] : 0.0;
         }
-        const CAS double* r2p = cptr(p.dc) + p2;
-        const CAS double* w2p = cptr(p.we) + p2;
-        const CAS double* d2p = cptr(p.dw) + p2;
+        const auto r2p = stream(p.dc + p2);
+        const auto w2p = stream(p.we + p2);
+        const auto d2p = stream(p.dw + p2);
         auto accumulate = [&](uint32_t bp, uint32_t bt, double pw, double pd) {
             if (bp < (uint32_t)NP && bt < (uint32_t)NT) {
                 if constexpr (CNT) {
@@ -216,11 +241,12 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
             auto block = [&](int j, auto UC) {
                 constexpr int UU = decltype(UC)::value;
                 double r2[UU], w2[UU], d2[UU];
+                const int jv = jidx(j);
 #pragma unroll
                 for (int u = 0; u < UU; ++u) {
-                    r2[u] = r2p[j + u];
-                    w2[u] = w2p[j + u];
-                    d2[u] = d2p[j + u];
+                    r2[u] = r2p[jv + u];
+                    w2[u] = w2p[jv + u];
+                    d2[u] = d2p[jv + u];
                 }
 #pragma unroll
                 for (int u = 0; u < UU; ++u) {
@@ -256,7 +282,7 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                 P1[k] = valid[k] ? (float)((r1[k] - base2) * cb) : 1048576.5f;  // 2^20 + 1/2: out of range
                 T1[k] = valid[k] ? (float)((r1[k] + base2) * sb + (double)tau) : 1048576.5f;
             }
-            const CAS float* o2p = cptr(p.off32) + p2;
+            const auto o2p = stream(p.off32 + p2);
             auto block = [&](int j, auto UC) {
                 constexpr int UU = decltype(UC)::value;
                 uint32_t bp[UU][PPL], bt[UU][PPL];
@@ -264,11 +290,12 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                 float o2v[UU];
                 double w2[UU], d2[UU];
                 float frmin = 1.0f;
+                const int jv = jidx(j);
 #pragma unroll
                 for (int u = 0; u < UU; ++u) {
-                    o2v[u] = o2p[j + u];
-                    w2[u] = w2p[j + u];
-                    d2[u] = d2p[j + u];
+                    o2v[u] = o2p[jv + u];
+                    w2[u] = w2p[jv + u];
+                    d2[u] = d2p[jv + u];
                 }
 #pragma unroll
                 for (int u = 0; u < UU; ++u) {
@@ -288,7 +315,7 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
 #pragma unroll
                         for (int k = 0; k < PPL; ++k) {
                             if (fr[u][k] < tau2) {
-                                const double r2 = r2p[j + u];
+                                const double r2 = r2p[jv + u];
                                 bp[u][k] = (uint32_t)__double2int_rz((fabs(r1[k] - r2) * c) * bp64);
                                 bt[u][k] = (uint32_t)__double2int_rz(((r1[k] + r2) * s) * bt64);
                             }
@@ -330,10 +357,10 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                 d1[k] = 0.f;
             }
         }
-        const CAS float* o2p = cptr(p.off32) + p2;
-        const CAS float* w2p = cptr(p.we32) + p2;
-        const CAS float* d2p = cptr(p.dw32) + p2;
-        const CAS double* r2p = cptr(p.dc) + p2;
+        const auto o2p = stream(p.off32 + p2);
+        const auto w2p = stream(p.we32 + p2);
+        const auto d2p = stream(p.dw32 + p2);
+        const auto r2p = stream(p.dc + p2);
         constexpr int U = 8;
         auto block = [&](int j, auto UC) {
             constexpr int UU = decltype(UC)::value;
@@ -341,11 +368,12 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
             float fr[UU][PPL];
             float o2v[UU], w2v[UU], d2v[UU];
             float frmin = 1.0f;
+            const int jv = jidx(j);
 #pragma unroll
             for (int u = 0; u < UU; ++u) {
-                o2v[u] = o2p[j + u];
-                w2v[u] = w2p[j + u];
-                d2v[u] = d2p[j + u];
+                o2v[u] = o2p[jv + u];
+                w2v[u] = w2p[jv + u];
+                d2v[u] = d2p[jv + u];
             }
 #pragma unroll
             for (int u = 0; u < UU; ++u) {
@@ -365,7 +393,7 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
 #pragma unroll
                     for (int k = 0; k < PPL; ++k) {
                         if (fr[u][k] < tau2) {
-                            const double r2 = r2p[j + u];
+                            const double r2 = r2p[jv + u];
                             bp[u][k] = (uint32_t)__double2int_rz((fabs(r1[k] - r2) * c) * bp64);
                             bt[u][k] = (uint32_t)__double2int_rz(((r1[k] + r2) * s) * bt64);
                         }
@@ -512,6 +540,375 @@ __global__ void __launch_bounds__(WAVES * 64) xi_kernel(const lya_xi_problem p, 
         }
         __syncthreads();
         flush_tile<MODE, CNT>(p, lds, R, ntpad, slot, tid, NTHR);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal walk (FP64).  The in-range pixel pairs (i, j) of a forest pair lie in
+// a band around the diagonals d = j - i.  Along one diagonal r1 and r2 grow
+// together: |r1 - r2| (the r_parallel bin) stays nearly constant and r1 + r2
+// (the r_perp bin) grows slowly, so consecutive pairs mostly share a bin (runs
+// of ~30 pairs at 4 Mpc/h bins, ~100 in-band diagonals per forest pair).  Lanes
+// walk diagonals with a wave-uniform forest-1 index i, keep the (w, dw, count)
+// sums of the current bin in registers and add them to the LDS histogram only
+// when the bin changes -- one LDS atomic per run instead of one per pair.
+//
+// Per item (forest 1 and a range of its neighbours), in batches of neighbours:
+//   1. one wave per neighbour: half-angles and the diagonal range [dlo, dhi)
+//      that can hold an in-range pair (per-pixel binary searches with the
+//      1e-3-bin margins of prune(), so skipped diagonals are out of range);
+//   2. sub-groups of 16 consecutive diagonals of one pair (their forest-2
+//      loads are coalesced); the span of a sub-group is the union of its
+//      diagonals' overlaps, each cut at the first step whose r1 + r2 exceeds
+//      the r_perp range (r1 + r2 grows along a diagonal of sorted forests);
+//   3. sub-groups sorted by (first step, last step) in LDS (bitonic);
+//   4. a wave walks four consecutive sorted sub-groups over the union of their
+//      spans; a lane outside its own diagonal's overlap reads forest-2 pixel 0
+//      and bins nothing.
+// ---------------------------------------------------------------------------
+#ifndef LYA_WALK_UNROLL
+#define LYA_WALK_UNROLL 8
+#endif
+constexpr int WALK_NB = 64;        // neighbours per batch
+constexpr int WALK_HS = 2688;      // LDS bins: (np_rp + 1) x (np_rt + 1), last row and column = trash
+constexpr int WALK_SG = 2048;      // sub-groups per batch
+constexpr int WALK_G = 16;         // diagonals per sub-group
+constexpr int WALK_MAX_LEN = 16000;  // longest forest: (2*len)/16 + 1 sub-groups fit a batch
+constexpr uint32_t WALK_NONE = 0xFFFFFFFFu;
+
+__device__ __forceinline__ int wave_min(int x) {
+    for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
+    return x;
+}
+__device__ __forceinline__ int wave_max(int x) {
+    for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
+    return x;
+}
+
+// Walk histogram in LDS: [cnt u32 | w f64 | dw f64] x WALK_HS, rows of ntpad =
+// np_rt + 1 entries; row np_rp and column np_rt take the runs of out-of-range
+// pairs and are never merged into the outputs.
+struct WalkLayout {
+    static constexpr uint32_t W = 4u * WALK_HS;
+    static constexpr uint32_t DW = 12u * WALK_HS;
+    static constexpr uint32_t BYTES = 20u * WALK_HS;
+};
+
+template <bool CNT>
+__device__ void flush_walk(const lya_xi_problem& p, unsigned char* lds, int ntpad, int slot, int tid, int nthr) {
+    using L = WalkLayout;
+    const int NT = p.np_rt;
+    const int nb_lds = p.np_rp * ntpad;
+    const int64_t gofs = (int64_t)slot * p.np_rp * NT;
+    for (int b = tid; b < nb_lds; b += nthr) {
+        uint32_t* hc = (uint32_t*)lds;
+        double* hw = (double*)(lds + L::W);
+        double* hd = (double*)(lds + L::DW);
+        const uint32_t cnt = hc[b];
+        const double sw = hw[b], sd = hd[b];
+        hc[b] = 0u;
+        hw[b] = 0.0;
+        hd[b] = 0.0;
+        const int bp = b / ntpad, bt = b - bp * ntpad;
+        if (bt < NT && (CNT ? (cnt != 0u) : (sw != 0.0 || sd != 0.0))) {
+            const int64_t g = gofs + (int64_t)bp * NT + bt;
+            if (CNT) atomicAdd((unsigned long long*)(p.out_cnt + g), (unsigned long long)cnt);
+            unsafeAtomicAdd(p.out_w + g, sw);
+            unsafeAtomicAdd(p.out_dw + g, sd);
+        }
+    }
+}
+
+template <int WAVES, bool CNT, bool POW2>
+__global__ void __launch_bounds__(WAVES * 64) xi_walk_kernel(const lya_xi_problem p, int ntpad) {
+    using L = WalkLayout;
+    __shared__ __align__(16) unsigned char lds[L::BYTES];
+    __shared__ uint64_t s_sg[WALK_SG];
+    __shared__ int64_t s_p2[WALK_NB];
+    __shared__ double s_cq[WALK_NB], s_sq[WALK_NB], s_rt[WALK_NB], s_dp[WALK_NB];
+    __shared__ int s_n2[WALK_NB], s_dlo[WALK_NB], s_dhi[WALK_NB], s_sgoff[WALK_NB], s_srt[WALK_NB];
+    __shared__ int s_tile, s_nq, s_total, s_next;
+    constexpr int NTHR = WAVES * 64;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = uni(tid >> 6);
+    const int NP = p.np_rp, NT = p.np_rt;
+    const double bp64 = (double)NP / p.rpmax;
+    const double bt64 = (double)NT / p.rtmax;
+
+    const uint32_t TRASH = (uint32_t)NP * (uint32_t)ntpad + (uint32_t)NT;
+
+    for (uint32_t o = (uint32_t)tid * 16u; o < L::BYTES; o += NTHR * 16u)
+        *(uint4*)(lds + o) = make_uint4(0u, 0u, 0u, 0u);
+    __syncthreads();
+
+    for (;;) {
+        if (tid == 0) s_tile = (int)atomicAdd(p.queue, 1u);
+        __syncthreads();
+        const int tile = uni(s_tile);
+        if (tile >= p.n_tiles) break;
+        const int ib = cptr(p.tiles)[2 * tile], ie = cptr(p.tiles)[2 * tile + 1];
+        const int slot = cptr(p.items)[4 * ib + 3];
+        for (int it = ib; it < ie; ++it) {
+            const int f1 = cptr(p.items)[4 * it + 0];
+            const int nb0 = cptr(p.items)[4 * it + 1];
+            const int nb1 = cptr(p.items)[4 * it + 2];
+            const int64_t p1 = cptr(p.pix_off)[f1];
+            const int n1 = (int)(cptr(p.pix_off)[f1 + 1] - p1);
+            const bool sorted1 = cptr(p.sorted)[f1] != 0;
+            for (int nbd = nb0; nbd < nb1;) {
+                const int batch = min(WALK_NB, nb1 - nbd);
+                // 1. half-angles of each neighbour (one thread each), then the diagonal
+                //    range [dlo, dhi) from 8-pixel slabs of forest 1 (all threads).  For
+                //    pixels i in a slab [i0, i1] every j with an in-range pair satisfies
+                //    jl(i0) <= j < min(jhp(i1), jrt(i0)), where jl counts r2 <= r1 - Dp,
+                //    jhp counts r2 < r1 + Dp and jrt counts r2 < Rt - r1 (jl and jhp grow
+                //    with i, jrt shrinks), so d = j - i lies in
+                //    [jl(i0) - i1, min(jhp(i1), jrt(i0)) - i0).  Dp and Rt carry the
+                //    1e-3-bin margins of prune(): skipped diagonals are out of range.
+                for (int q = tid; q < batch; q += NTHR) {
+                    const int f2 = p.nb_idx[nbd + q];
+                    const int64_t p2 = p.pix_off[f2];
+                    const int n2 = (int)(p.pix_off[f2 + 1] - p2);
+                    double c, s;
+                    halfangles(p.ux[f1], p.uy[f1], p.uz[f1], p.ux[f2], p.uy[f2], p.uz[f2], c, s);
+                    const bool both_sorted = sorted1 && p.sorted[f2] != 0;
+                    s_p2[q] = p2;
+                    s_n2[q] = n2;
+                    s_cq[q] = POW2 ? c * bp64 : c;
+                    s_sq[q] = POW2 ? s * bt64 : s;
+                    s_dp[q] = ((double)NP + 1e-3) / (c * bp64);
+                    s_rt[q] = ((double)NT + 1e-3) / (s * bt64);
+                    s_srt[q] = both_sorted;
+                    s_dlo[q] = both_sorted ? INT_MAX : -(n1 - 1);
+                    s_dhi[q] = both_sorted ? INT_MIN : n2;
+                }
+                __syncthreads();
+                {
+                    const int nsl = (n1 + 7) >> 3;
+                    for (int u = tid; u < batch * nsl; u += NTHR) {
+                        const int q = u / nsl, sl = u - q * nsl;
+                        if (!s_srt[q]) continue;
+                        const int i0 = sl << 3, i1 = min(i0 + 7, n1 - 1);
+                        const int n2 = s_n2[q];
+                        const double* r2p = p.dc + s_p2[q];
+                        const double ra = p.dc[p1 + i0], rb = p.dc[p1 + i1];
+                        const double lo = ra - s_dp[q], hp = rb + s_dp[q], hr = s_rt[q] - ra;
+                        int a0 = 0, b0 = n2, a1 = 0, b1 = n2, a2 = 0, b2 = n2;
+                        while (a0 < b0 || a1 < b1 || a2 < b2) {
+                            if (a0 < b0) {
+                                const int m = (a0 + b0) >> 1;
+                                if (r2p[m] <= lo) a0 = m + 1; else b0 = m;
+                            }
+                            if (a1 < b1) {
+                                const int m = (a1 + b1) >> 1;
+                                if (r2p[m] < hp) a1 = m + 1; else b1 = m;
+                            }
+                            if (a2 < b2) {
+                                const int m = (a2 + b2) >> 1;
+                                if (r2p[m] < hr) a2 = m + 1; else b2 = m;
+                            }
+                        }
+                        const int jl = a0, jh = min(a1, a2);
+                        if (jh > jl) {
+                            atomicMin(&s_dlo[q], jl - i1);
+                            atomicMax(&s_dhi[q], jh - i0);
+                        }
+                    }
+                }
+                __syncthreads();
+                // neighbours whose sub-groups fit the batch (prefix), and their offsets
+                if (wave == 0) {
+                    const int cnt = (lane < batch && s_dhi[lane] > s_dlo[lane])
+                                        ? (s_dhi[lane] - s_dlo[lane] + WALK_G - 1) / WALK_G : 0;
+                    int inc = cnt;
+                    for (int o = 1; o < 64; o <<= 1) {
+                        const int v = __shfl_up(inc, o, 64);
+                        if (lane >= o) inc += v;
+                    }
+                    const bool fits = lane < batch && inc <= WALK_SG;
+                    const int nq = max(1, __popcll(__ballot(fits)));
+                    if (lane < batch) s_sgoff[lane] = inc - cnt;
+                    const int total = __shfl(inc, nq - 1, 64);
+                    if (lane == 0) {
+                        s_nq = nq;
+                        s_total = min(total, WALK_SG);
+                    }
+                }
+                __syncthreads();
+                const int nq = s_nq, total = s_total;
+                // 2. sub-group spans: four sub-groups per wave, one lane per diagonal
+                for (int g0 = wave * 4; g0 < total; g0 += WAVES * 4) {
+                    const int g = g0 + (lane >> 4), k = lane & 15;
+                    int q = 0;
+                    for (int b = 32; b > 0; b >>= 1)  // last neighbour with sgoff <= g
+                        if (q + b < nq && s_sgoff[q + b] <= g) q += b;
+                    const int d0 = s_dlo[q] + (g - s_sgoff[q]) * WALK_G;
+                    int tlo = INT_MAX, thi = 0;
+                    if (g < total) {
+                        const int d = d0 + k;
+                        const int n2 = s_n2[q];
+                        if (d < s_dhi[q]) {
+                            const int a = max(0, -d);
+                            int b = min(n1, n2 - d);
+                            if (b > a && s_srt[q]) {
+                                const double Rt = s_rt[q];
+                                const int64_t p2d = s_p2[q] + d;
+                                int lo = a, hi = b;  // first step with r1 + r2 >= Rt
+                                while (lo < hi) {
+                                    const int m = (lo + hi) >> 1;
+                                    if (p.dc[p1 + m] + p.dc[p2d + m] >= Rt) hi = m; else lo = m + 1;
+                                }
+                                b = lo;
+                            }
+                            if (b > a) {
+                                tlo = a;
+                                thi = b;
+                            }
+                        }
+                    }
+                    for (int o = 8; o > 0; o >>= 1) {
+                        tlo = min(tlo, __shfl_xor(tlo, o, 16));
+                        thi = max(thi, __shfl_xor(thi, o, 16));
+                    }
+                    if (k == 0 && g < total) {
+                        const uint64_t t0 = thi > tlo ? (uint64_t)tlo : 0ull, t1 = thi > tlo ? (uint64_t)thi : 0ull;
+                        s_sg[g] = (t0 << 48) | (t1 << 32) | ((uint64_t)q << 16) | (uint64_t)(uint32_t)(d0 + 32768);
+                    }
+                }
+                __syncthreads();
+                // 3. sort the sub-groups by (first step, last step)
+                int pw = 1;
+                while (pw < total) pw <<= 1;
+                for (int t = total + tid; t < pw; t += NTHR) s_sg[t] = ~0ull;
+                __syncthreads();
+                for (int kk = 2; kk <= pw; kk <<= 1)
+                    for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+                        for (int t = tid; t < pw; t += NTHR) {
+                            const int u = t ^ jj;
+                            if (u > t) {
+                                const uint64_t x = s_sg[t], y = s_sg[u];
+                                if ((x > y) == ((t & kk) == 0)) {
+                                    s_sg[t] = y;
+                                    s_sg[u] = x;
+                                }
+                            }
+                        }
+                        __syncthreads();
+                    }
+                if (tid == 0) s_next = 0;
+                __syncthreads();
+                // 4. walk
+                for (;;) {
+                    int gw = 0;
+                    if (lane == 0) gw = atomicAdd(&s_next, 1);
+                    const int gb = 4 * uni(gw);
+                    if (gb >= total) break;
+                    const int h = lane >> 4, k = lane & 15;
+                    const uint64_t e = (gb + h < total) ? s_sg[gb + h] : 0ull;
+                    const int tmin = (int)(e >> 48), tmax = (int)((e >> 32) & 0xFFFFu);
+                    const int q = (int)((e >> 16) & 0xFFu), d = (int)(e & 0xFFFFu) - 32768 + k;
+                    const bool ne = tmax > tmin;
+                    const int x0 = ne ? tmin : INT_MAX, x1 = ne ? tmax : 0;
+                    const int T0 = min(min(__builtin_amdgcn_readlane(x0, 0), __builtin_amdgcn_readlane(x0, 16)),
+                                       min(__builtin_amdgcn_readlane(x0, 32), __builtin_amdgcn_readlane(x0, 48)));
+                    const int T1 = max(max(__builtin_amdgcn_readlane(x1, 0), __builtin_amdgcn_readlane(x1, 16)),
+                                       max(__builtin_amdgcn_readlane(x1, 32), __builtin_amdgcn_readlane(x1, 48)));
+                    if (T1 <= T0) continue;
+                    const bool valid = ne && d < s_dhi[q];
+                    const int n2 = s_n2[q];
+                    const int a = valid ? max(0, -d) : 0;
+                    const int b = valid ? min(n1, n2 - d) : 0;
+                    // forest-2 pixel of step t: jb + t.  Every step of the wave lies in
+                    // [0, n1) and -n1 < d < n2, so jb + t stays within [-pad, n_pix + pad):
+                    // the byte offset is fixed per lane and the step moves the uniform base.
+                    // Lanes without a diagonal read the padding, whose dc sentinel puts
+                    // every pair in a trash bin.
+                    const int64_t jb = valid ? s_p2[q] + d : -(int64_t)p.pad;
+                    const int64_t voff = (jb + p.pad) * 8;
+                    const double cq = s_cq[q], sq = s_sq[q];
+                    // steps where every lane with a diagonal is inside its overlap
+                    const int A = wave_max(valid ? a : INT_MIN), B = wave_min(valid ? b : INT_MAX);
+                    const int M0 = max(T0, min(A, T1)), M1 = max(M0, min(T1, B));
+                    uint32_t prev = TRASH, an = 0u;
+                    double aw = 0.0, ad = 0.0;
+                    const char* dcb = (const char*)(p.dc - p.pad);
+                    const char* web = (const char*)(p.we - p.pad);
+                    const char* dwb = (const char*)(p.dw - p.pad);
+                    auto step = [&](auto CHK, int t, double r1, double w1, double d1, double r2, double w2,
+                                    double d2) {
+                        double qp = fabs(r1 - r2) * cq;
+                        double qt = (r1 + r2) * sq;
+                        if constexpr (!POW2) {
+                            qp = qp * bp64;
+                            qt = qt * bt64;
+                        }
+                        // saturating conversions: out-of-range rows and columns clamp to the trash
+                        const uint32_t kp = min((uint32_t)__double2int_rz(qp), (uint32_t)NP);
+                        const uint32_t kt = min((uint32_t)__double2int_rz(qt), (uint32_t)NT);
+                        uint32_t key = __umul24(kp, (uint32_t)ntpad) + kt;
+                        if constexpr (decltype(CHK)::value) key = ((t >= a) & (t < b)) ? key : TRASH;
+                        const double pw = w1 * w2, pd = d1 * d2;
+                        if (key != prev) {  // the run of `prev` ended
+                            if constexpr (CNT) lds_add<uint32_t>(lds, 4u * prev, an);
+                            lds_add<double>(lds, L::W + 8u * prev, aw);
+                            lds_add<double>(lds, L::DW + 8u * prev, ad);
+                            aw = 0.0;
+                            ad = 0.0;
+                            an = 0u;
+                        }
+                        aw += pw;
+                        ad += pd;
+                        if constexpr (CNT) ++an;
+                        prev = key;
+                    };
+                    const CAS double* r1p = cptr(p.dc) + p1;
+                    const CAS double* w1p = cptr(p.we) + p1;
+                    const CAS double* d1p = cptr(p.dw) + p1;
+                    // a block's loads are all issued before its first step, so one memory
+                    // latency is exposed per block rather than per step
+                    auto block = [&](auto CHK, int t, auto UC) {
+                        constexpr int UU = decltype(UC)::value;
+                        double r2[UU], w2[UU], d2[UU], r1[UU], w1[UU], d1[UU];
+                        const int64_t tb = (int64_t)t * 8 + voff;
+#pragma unroll
+                        for (int u = 0; u < UU; ++u) {
+                            r2[u] = *(const double*)(dcb + tb + 8 * u);
+                            w2[u] = *(const double*)(web + tb + 8 * u);
+                            d2[u] = *(const double*)(dwb + tb + 8 * u);
+                        }
+#pragma unroll
+                        for (int u = 0; u < UU; ++u) {
+                            r1[u] = r1p[t + u];
+                            w1[u] = w1p[t + u];
+                            d1[u] = d1p[t + u];
+                        }
+#pragma unroll
+                        for (int u = 0; u < UU; ++u) step(CHK, t + u, r1[u], w1[u], d1[u], r2[u], w2[u], d2[u]);
+                    };
+                    constexpr int U = LYA_WALK_UNROLL;
+                    using Blk = std::integral_constant<int, U>;
+                    using One = std::integral_constant<int, 1>;
+                    int t = T0;
+                    for (; t + U <= T1; t += U) {
+                        // only blocks reaching outside [M0, M1) need the per-lane overlap test
+                        if (t < M0 || t + U > M1)
+                            block(std::true_type{}, t, Blk{});
+                        else
+                            block(std::false_type{}, t, Blk{});
+                    }
+                    for (; t < T1; ++t) block(std::true_type{}, t, One{});
+                    if constexpr (CNT) lds_add<uint32_t>(lds, 4u * prev, an);
+                    lds_add<double>(lds, L::W + 8u * prev, aw);
+                    lds_add<double>(lds, L::DW + 8u * prev, ad);
+                }
+                __syncthreads();  // tables and sub-groups are rewritten by the next batch
+                nbd += nq;
+            }
+        }
+        __syncthreads();
+        flush_walk<CNT>(p, lds, ntpad, slot, tid, NTHR);
         __syncthreads();
     }
 }
@@ -669,6 +1066,13 @@ XiKernel pick_c(bool cnt, int R, int ppl, int waves) {
     return nullptr;
 }
 
+template <bool CNT, bool POW2>
+XiKernel pick_walk(int waves) {
+    if (waves == 8) return xi_walk_kernel<8, CNT, POW2>;
+    if (waves == 16) return xi_walk_kernel<16, CNT, POW2>;
+    return nullptr;
+}
+
 bool is_pow2(double x) {
     int e = 0;
     return x > 0 && std::frexp(x, &e) == 0.5;
@@ -692,7 +1096,17 @@ int resolve(const lya_xi_problem* p, const lya_xi_config* cfg, Resolved& r) {
     if (r.R > r.waves) return lya_fail(LYA_ERR_ARG, "replicas > waves");
     const bool cnt = !(cfg && cfg->no_counts);
     const bool pow2 = p && is_pow2((double)p->np_rp / p->rpmax) && is_pow2((double)p->np_rt / p->rtmax);
-    if (r.mode == LYA_MODE_FP64)
+    const bool walk = cfg && cfg->walk;
+    if (walk) {
+        if (r.mode != LYA_MODE_FP64) return lya_fail(LYA_ERR_UNSUPPORTED, "walk 1 runs the FP64 mode only");
+        if (p && p->max_len > WALK_MAX_LEN)
+            return lya_fail(LYA_ERR_UNSUPPORTED, "walk 1: forests of %d pixels (> %d)", p->max_len, WALK_MAX_LEN);
+        if (p && (p->pad < p->max_len || p->max_len < 0))
+            return lya_fail(LYA_ERR_ARG, "walk 1 needs pad (%d) >= max_len (%d)", p->pad, p->max_len);
+        r.R = 1;
+        r.fn = pow2 ? (cnt ? pick_walk<true, true>(r.waves) : pick_walk<false, true>(r.waves))
+                    : (cnt ? pick_walk<true, false>(r.waves) : pick_walk<false, false>(r.waves));
+    } else if (r.mode == LYA_MODE_FP64)
         r.fn = pow2 ? pick_c<LYA_MODE_FP64, true>(cnt, r.R, r.ppl, r.waves)
                     : pick_c<LYA_MODE_FP64, false>(cnt, r.R, r.ppl, r.waves);
     else if (r.mode == LYA_MODE_MIXED)
@@ -703,7 +1117,10 @@ int resolve(const lya_xi_problem* p, const lya_xi_config* cfg, Resolved& r) {
     if (p) {
         if (p->np_rp < 1 || p->np_rt < 1 || !(p->rpmax > 0) || !(p->rtmax > 0))
             return lya_fail(LYA_ERR_ARG, "bad binning");
-        r.ntpad = p->np_rt + pad;
+        r.ntpad = p->np_rt + (walk ? 1 : pad);
+        if (walk && (int64_t)(p->np_rp + 1) * r.ntpad > WALK_HS)
+            return lya_fail(LYA_ERR_UNSUPPORTED, "walk 1: (np_rp+1)*(np_rt+1)=%d exceeds %d LDS bins",
+                            (p->np_rp + 1) * r.ntpad, WALK_HS);
         if ((int64_t)p->np_rp * r.ntpad > HS)
             return lya_fail(LYA_ERR_UNSUPPORTED, "np_rp*(np_rt+pad)=%d exceeds LDS histogram capacity %d",
                             p->np_rp * r.ntpad, HS);
@@ -712,6 +1129,7 @@ int resolve(const lya_xi_problem* p, const lya_xi_config* cfg, Resolved& r) {
     }
     r.lds = 16 + ((r.mode == LYA_MODE_MIXED_F32) ? Layout<LYA_MODE_MIXED_F32>::bytes(r.R)
                                                  : Layout<LYA_MODE_FP64>::bytes(r.R));  // static LDS
+    if (walk) r.lds = 16 + WalkLayout::BYTES + WALK_SG * 8 + WALK_NB * (8 * 5 + 4 * 5);
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)

@@ -19,6 +19,8 @@ from . import _lib
 from ._lib import XiProblem, XiConfig, LYA_MODE_FP64, LYA_MODE_MIXED, LYA_MODE_MIXED_F32, check, lib, ptr
 
 MODES = {"fp64": LYA_MODE_FP64, "mixed": LYA_MODE_MIXED, "mixed32": LYA_MODE_MIXED_F32}
+# dc of the padding around the pixel arrays: any pair with it falls outside the bins
+DC_PAD = 1e300
 
 
 def build_pairs(forests, angmax):
@@ -148,6 +150,7 @@ def default_config(mode="mixed", **kw):
     cfg.groups = kw.get("groups", 0)
     cfg.no_counts = 0 if kw.get("counts", True) else 1
     cfg.lane_map = kw.get("lane_map", 1)
+    cfg.walk = kw.get("walk", 0)
     return cfg
 
 
@@ -242,13 +245,23 @@ class XiEngine:
             t = t.to(dtype)
         return t.to(self.device)
 
+    def _padded(self, a, fill=0.0):
+        """Device copy of a pixel array with ``self.pad`` entries of ``fill`` on both
+        sides (the diagonal walk reads up to one forest length outside the forests);
+        returns the view of the data, whose first element is pixel 0."""
+        buf = self.torch.full((len(a) + 2 * self.pad,), fill, dtype=self.torch.float64, device=self.device)
+        buf[self.pad:self.pad + len(a)] = self.torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64))
+        return buf[self.pad:self.pad + len(a)]
+
     def _upload(self):
         torch = self.torch
         f = self.forests
+        self.max_len = int(np.max(f.n)) if f.nf else 0
+        self.pad = max(1, self.max_len)
         self.d = {
             "pix_off": self._t(f.pix_off), "ux": self._t(f.ux), "uy": self._t(f.uy), "uz": self._t(f.uz),
             "base": self._t(f.base), "sorted": self._t(f.sorted),
-            "dc": self._t(f.dc), "we": self._t(f.we), "dw": self._t(f.dw),
+            "dc": self._padded(f.dc, DC_PAD), "we": self._padded(f.we), "dw": self._padded(f.dw),
             "off32": self._t(f.off32), "we32": self._t(f.we32), "dw32": self._t(f.dw32),
         }
         if self.pair_builder == "host":
@@ -303,6 +316,8 @@ class XiEngine:
         p.np_rp, p.np_rt, p.rpmax, p.rtmax = self.np_rp, self.np_rt, self.rpmax, self.rtmax
         p.tau = self.tau
         p.lane_map = getattr(cfg, "lane_map", 1)
+        p.max_len = self.max_len
+        p.pad = self.pad
         p.out_cnt, p.out_w, p.out_dw, p.queue = ptr(self.out_cnt), ptr(self.out_w), ptr(self.out_dw), ptr(self.queue)
         self._keep = (items, tiles)
         return p
