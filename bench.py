@@ -153,7 +153,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--only", choices=["mixed", "fp64", "mixed32"], default=None,
+    ap.add_argument("--counts", action="store_true", help="time with per-bin pair counts on")
+    ap.add_argument("--only", choices=["mixed", "fp64", "fp64-walk", "mixed32"], default=None,
                     help="time one kernel only (profiling)")
     ap.add_argument("--ppl", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
@@ -208,27 +209,41 @@ def main():
     log("setup: %s" % setup)
 
     def shape_for(mode):
-        best = autotune.load_best(mode)
-        shape = {k: v for k, v in (best or {}).items()
-                 if k in ("px_per_lane", "waves", "replicas", "row_pad", "lane_map")}
+        walk = 1 if mode == "fp64-walk" else (0 if mode == "fp64" else None)
+        base = "fp64" if mode.startswith("fp64") else mode
+        shape = dict(autotune.load_best(base, cfgname, walk=walk) or {})
+        if mode == "fp64-walk":
+            shape = {"walk": 1, "waves": shape.get("waves", 16)}
         for k, a in (("px_per_lane", args.ppl), ("waves", args.waves), ("replicas", args.replicas)):
-            if a:
+            if a and not shape.get("walk"):
                 shape[k] = a
         if args.row_pad >= 0:
             shape["row_pad"] = args.row_pad
-        return shape
+        return base, shape
 
+    # Timed launches compute the reference's outputs (sum w1 w2, sum dw1 dw2); one
+    # further untimed launch per kernel adds the per-bin pair counts for the
+    # bit-exact comparison between kernels.
     out = {}
-    modes = [args.only] if args.only else (["mixed"] if args.no_fp64 else ["mixed", "fp64"])
+    modes = [args.only] if args.only else (["mixed"] if args.no_fp64 else ["mixed", "fp64", "fp64-walk"])
     for mode in modes:
-        shape = shape_for(mode)
-        g = eng.graph(xi.default_config(mode, **shape))
+        base, shape = shape_for(mode)
+        g = eng.graph(xi.default_config(base, counts=args.counts, **shape))
         wall, ev = time_graph(g, eng, args.steps, args.warmup, torch)
         res = eng.results()
         g.close()
-        out[mode] = {"wall": wall, "ev": ev, "res": res, "shape": shape}
-        log("%s: %.4f s/step (events median %.4f s) -> %.4g pixel-pairs/s"
-            % (mode, wall, float(np.median(ev)), P / wall))
+        t_cnt = None
+        if not args.counts:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            eng.run(xi.default_config(base, counts=True, **shape))
+            e1.record()
+            torch.cuda.synchronize()
+            t_cnt = e0.elapsed_time(e1) / 1e3
+            res = eng.results()
+        out[mode] = {"wall": wall, "ev": ev, "res": res, "shape": shape, "with_counts_s": t_cnt}
+        log("%s %s: %.4f s/step (events median %.4f s) -> %.4g pixel-pairs/s"
+            % (mode, shape, wall, float(np.median(ev)), P / wall))
 
     if args.copy_calibrate:
         from lya2pcf_amd import _lib
@@ -245,22 +260,29 @@ def main():
     value = P / m["wall"]
     ev_med = float(np.median(m["ev"]))
     achieved = P * FLOP_PER_PAIR / ev_med / 1e12
-    peak = FP64_PEAK_TFLOPS if main_mode == "fp64" else FP32_PEAK_TFLOPS
+    peak = FP64_PEAK_TFLOPS if main_mode.startswith("fp64") else FP32_PEAK_TFLOPS
     line = {
         "metric": METRIC, "value": value, "unit": "pixel-pairs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": m["wall"] * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64" if main_mode == "fp64" else "f32+f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f64" if main_mode.startswith("fp64") else "f32+f64", "data": "synthetic",
         "config": {"workload": cfg.name, "forests": fs.nf, "pixels": fs.n_pix, "forest_pairs": int(eng.n_pairs),
-                   "pixel_pairs": P, "bins": "50x50 of 4 Mpc/h", "kernel": main_mode, "shape": out[main_mode]["shape"],
+                   "pixel_pairs": P, "bins": "50x50 of 4 Mpc/h", "kernel": main_mode, "shape": m["shape"],
+                   "outputs": "sum w1w2, sum dw1dw2" + (" and pair counts" if args.counts else ""),
                    "parallelism": "single GPU, persistent work queue"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": autotune.measured_traffic(main_mode),
                      "note": "10 FLOP per pixel pair (algorithmic) / median per-launch hipEvent time; "
                              "no MFMA or HBM bound applies"},
+        "kernels": {k: {"value": P / v["wall"], "ms_per_step": v["wall"] * 1e3,
+                        "event_median_ms": float(np.median(v["ev"])) * 1e3, "shape": v["shape"],
+                        "with_counts_ms": None if v["with_counts_s"] is None else v["with_counts_s"] * 1e3}
+                    for k, v in out.items()},
         "setup": setup,
     }
-    if "fp64" in out and "mixed" in out:
-        c64, w64, d64 = out["fp64"]["res"]
+    fp64s = [k for k in ("fp64", "fp64-walk") if k in out]
+    if fp64s and "mixed" in out:
+        best64 = min(fp64s, key=lambda k: out[k]["wall"])
+        c64, w64, d64 = out[best64]["res"]
         cm, wm, dm = out["mixed"]["res"]
         with np.errstate(divide="ignore", invalid="ignore"):
             x64 = np.where(w64 > 0, d64 / w64, 0.0)
@@ -269,10 +291,12 @@ def main():
         X64 = np.where(W64 > 0, d64.sum(0) / np.where(W64 > 0, W64, 1), 0.0)
         Xm = np.where(Wm > 0, dm.sum(0) / np.where(Wm > 0, Wm, 1), 0.0)
         sel = np.abs(X64) > 0
-        line["fp64"] = {"value": P / out["fp64"]["wall"], "ms_per_step": out["fp64"]["wall"] * 1e3,
-                        "shape": out["fp64"]["shape"],
-                        "speedup_mixed_over_fp64": out["fp64"]["wall"] / out["mixed"]["wall"],
+        line["fp64"] = {"kernel": best64, "value": P / out[best64]["wall"],
+                        "ms_per_step": out[best64]["wall"] * 1e3, "shape": out[best64]["shape"],
+                        "speedup_mixed_over_fp64": out[best64]["wall"] / out["mixed"]["wall"],
                         "counts_bit_exact": bool(np.array_equal(c64, cm)),
+                        "counts_bit_exact_all_kernels": bool(all(np.array_equal(out[k]["res"][0], cm)
+                                                                 for k in fp64s)),
                         "xi_total_max_rel_err": float(np.max(np.abs(Xm - X64)[sel] / np.abs(X64[sel]))),
                         "xi_per_pixel_max_abs_err": float(np.max(np.abs(xm - x64)))}
     if "mixed32" in out and "fp64" in out:

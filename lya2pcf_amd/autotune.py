@@ -1,18 +1,22 @@
 """Autotuner for the persistent pair-binning kernel.
 
 Search space (all shapes give identical bins):
-  px_per_lane  1, 2, 4   -> forest-1 tile of 64, 128, 256 pixels per wave
-  replicas     1, 2      -> LDS histogram copies (waves w, w+R, ... share one)
-  row_pad      0, 1      -> LDS row stride np_rt or np_rt + 1 (bank padding)
-  waves        4, 8, 16  -> waves per workgroup
-  lane_map     0, 1, 2   -> lane order of forest-1 pixels (consecutive, by 2, by 4)
+  walk 0 (lanes hold forest-1 pixels, every mode):
+    px_per_lane  1, 2, 4   -> forest-1 tile of 64, 128, 256 pixels per wave
+    replicas     1, 2      -> LDS histogram copies (waves w, w+R, ... share one)
+    row_pad      0, 1      -> LDS row stride np_rt or np_rt + 1 (bank padding)
+    waves        4, 8, 16  -> waves per workgroup
+    lane_map     0, 1, 2   -> lane order of forest-1 pixels (consecutive, by 2, by 4)
+  walk 1 (lanes walk diagonals, FP64 only):
+    waves        8, 16
 The default sweep fixes replicas=1 and row_pad=1 (a second LDS replica halves
 occupancy; the padding removes bank conflicts: both measured in a full sweep).
-Each candidate is timed as hipGraph replays of the whole problem; the best
-shape per mode is written to ``tuned.json`` next to this file and used as the
-default by the benchmark.
+Each candidate is timed as hipGraph replays of the whole problem, with or
+without the per-bin pair counts; the best shape per workload and mode is
+written to ``tuned.json`` next to this file and used by the benchmark and the
+drop-in procedures.
 
-    python -m lya2pcf_amd.autotune [--forests N] [--modes mixed,fp64]
+    python -m lya2pcf_amd.autotune [--config c1] [--forests N] [--modes mixed,fp64] [--counts]
 """
 import itertools
 import json
@@ -25,14 +29,24 @@ CACHE = os.path.join(HERE, "tuned.json")
 PROFILES = os.path.join(os.path.dirname(HERE), "profiles")
 
 SPACE = {"px_per_lane": (1, 2, 4), "replicas": (1,), "row_pad": (1,), "waves": (8, 16), "lane_map": (0, 1, 2)}
+WALK_SPACE = {"walk": (1,), "waves": (8, 16)}
+SHAPE_KEYS = ("px_per_lane", "waves", "replicas", "row_pad", "lane_map", "walk")
 
 
-def load_best(mode="mixed"):
+def load_best(mode="mixed", config="c1", walk=None):
+    """Tuned kernel shape of ``mode`` for workload ``config`` (falls back to c1);
+    with ``walk`` = 0 or 1, the best shape of that kernel family."""
     try:
         with open(CACHE) as f:
-            return json.load(f).get(mode, {}).get("shape")
+            t = json.load(f)
     except (OSError, ValueError):
         return None
+    entry = (t.get(config) or t.get("c1") or {}).get(mode, {})
+    if walk is None:
+        best = entry.get("shape")
+    else:
+        best = next((r["shape"] for r in entry.get("all", []) if r["shape"].get("walk", 0) == walk), None)
+    return {k: v for k, v in best.items() if k in SHAPE_KEYS} if best else None
 
 
 def measured_traffic(mode="mixed"):
@@ -45,23 +59,24 @@ def measured_traffic(mode="mixed"):
         return None
 
 
-def candidates(space=None):
-    space = space or SPACE
-    keys = list(space)
-    for vals in itertools.product(*space.values()):
-        c = dict(zip(keys, vals))
-        if c.get("replicas", 1) > c.get("waves", 8):
-            continue
-        yield c
+def candidates(mode, space=None):
+    spaces = [space or SPACE] + ([WALK_SPACE] if mode == "fp64" and space is None else [])
+    for sp in spaces:
+        keys = list(sp)
+        for vals in itertools.product(*sp.values()):
+            c = dict(zip(keys, vals))
+            if c.get("replicas", 1) > c.get("waves", 8):
+                continue
+            yield c
 
 
-def tune(eng, mode, reps=3, log=print):
+def tune(eng, mode, reps=3, log=print, counts=False):
     import torch
     from . import xi
     results = []
-    for shape in candidates():
+    for shape in candidates(mode):
         try:
-            g = eng.graph(xi.default_config(mode, **shape))
+            g = eng.graph(xi.default_config(mode, counts=counts, **shape))
         except Exception as e:  # shape not supported (LDS size)
             log("skip %s: %s" % (shape, e))
             continue
@@ -88,6 +103,7 @@ def main():
     ap.add_argument("--forests", type=int, default=30000)
     ap.add_argument("--modes", default="mixed,fp64")
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--counts", action="store_true", help="tune with per-bin pair counts on")
     a = ap.parse_args()
     cfg = synth.scaled(synth.CONFIGS[a.config], a.forests)
     fs = forests.ForestSet.from_survey(synth.generate(cfg), float32_inputs=True)
@@ -95,13 +111,21 @@ def main():
     print("tuning on %d forests, %.3g pixel pairs" % (fs.nf, eng.pixel_pairs), flush=True)
     out = {}
     for mode in a.modes.split(","):
-        res = tune(eng, mode, a.reps, log=lambda s: print(s, flush=True))
+        res = tune(eng, mode, a.reps, log=lambda s: print(s, flush=True), counts=a.counts)
         out[mode] = {"shape": res[0][1], "seconds": res[0][0], "pixel_pairs": eng.pixel_pairs,
-                     "workload": cfg.name + " (%d forests)" % fs.nf,
+                     "counts": a.counts, "workload": cfg.name + " (%d forests)" % fs.nf,
                      "all": [{"shape": s, "seconds": t} for t, s in res]}
     path = os.environ.get("LYA_TUNE_OUT", CACHE)
+    try:
+        with open(path) as f:
+            table = json.load(f)
+        if "mixed" in table or "fp64" in table:  # older single-workload table
+            table = {}
+    except (OSError, ValueError):
+        table = {}
+    table.setdefault(a.config, {}).update(out)
     with open(path, "w") as f:
-        json.dump(out, f, indent=1)
+        json.dump(table, f, indent=1)
     print("wrote", path)
 
 

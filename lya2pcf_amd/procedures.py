@@ -67,9 +67,7 @@ def _compute():
         # the tuned kernel shape of this precision, if a tuning table is present
         from . import autotune
         shape = autotune.load_best(_S["precision"]) or {}
-        cfg = _xi.default_config(_S["precision"], counts=_S["counts"],
-                                 **{k: v for k, v in shape.items()
-                                    if k in ("px_per_lane", "waves", "replicas", "row_pad", "lane_map")})
+        cfg = _xi.default_config(_S["precision"], counts=_S["counts"], **shape)
     g = eng.graph(cfg)
     g.launch()
     _S["cnt"], _S["w"], _S["dw"] = eng.results()
