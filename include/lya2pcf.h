@@ -165,6 +165,9 @@ typedef struct lya_xi_config {
     int32_t walk;             /* 0: lanes hold forest-1 pixels (all modes);           */
                               /* 1: lanes walk diagonals (i, i+d) of each forest pair */
                               /*    and sum runs of equal bins in registers (FP64)    */
+    int32_t merge;            /* walk 0, fp64 and mixed: 1 sums runs of equal bins    */
+                              /* within each block of forest-2 pixels before the LDS  */
+                              /* atomics (one atomic set per run)                     */
 } lya_xi_config;
 
 /* Largest mixed-mode error bound for these data (host): returns tau in bin units. */

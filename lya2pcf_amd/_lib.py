@@ -39,7 +39,8 @@ class XiConfig(C.Structure):
     """Mirror of ``lya_xi_config`` (``lane_map`` is a Python-side field copied into
     the problem's ``lane_map``)."""
     _fields_ = [("mode", _i32), ("px_per_lane", _i32), ("waves", _i32), ("replicas", _i32),
-                ("row_pad", _i32), ("groups", _i32), ("no_counts", _i32), ("walk", _i32)]
+                ("row_pad", _i32), ("groups", _i32), ("no_counts", _i32), ("walk", _i32),
+                ("merge", _i32)]
     lane_map = 1
 
     def as_dict(self):

@@ -7,10 +7,12 @@ Search space (all shapes give identical bins):
     row_pad      0, 1      -> LDS row stride np_rt or np_rt + 1 (bank padding)
     waves        4, 8, 16  -> waves per workgroup
     lane_map     0, 1, 2   -> lane order of forest-1 pixels (consecutive, by 2, by 4)
+    merge        0, 1      -> sum runs of equal bins per block before the LDS atomics
   walk 1 (lanes walk diagonals, FP64 only):
     waves        8, 16
-The default sweep fixes replicas=1 and row_pad=1 (a second LDS replica halves
-occupancy; the padding removes bank conflicts: both measured in a full sweep).
+The default sweep fixes replicas=1 and row_pad=1 and skips lane_map 0 (a second
+LDS replica halves occupancy; the padding removes bank conflicts; consecutive lanes
+are 33 % slower: all measured in a full sweep).
 Each candidate is timed as hipGraph replays of the whole problem, with or
 without the per-bin pair counts; the best shape per workload and mode is
 written to ``tuned.json`` next to this file and used by the benchmark and the
@@ -28,9 +30,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuned.json")
 PROFILES = os.path.join(os.path.dirname(HERE), "profiles")
 
-SPACE = {"px_per_lane": (1, 2, 4), "replicas": (1,), "row_pad": (1,), "waves": (8, 16), "lane_map": (0, 1, 2)}
+SPACE = {"px_per_lane": (1, 2, 4), "replicas": (1,), "row_pad": (1,), "waves": (8, 16), "lane_map": (1, 2),
+         "merge": (0, 1)}
 WALK_SPACE = {"walk": (1,), "waves": (8, 16)}
-SHAPE_KEYS = ("px_per_lane", "waves", "replicas", "row_pad", "lane_map", "walk")
+SHAPE_KEYS = ("px_per_lane", "waves", "replicas", "row_pad", "lane_map", "walk", "merge")
 
 
 def load_best(mode="mixed", config="c1", walk=None):

@@ -175,7 +175,7 @@ __device__ __forceinline__ void prune(const lya_xi_problem& p, int f1, int64_t p
 //         half-angle factors, fl(|d| (c b)) = fl(|d| c) b exactly when b = 2^k,
 //         so the bin equals the reference's floor(fl(fl(|d| c) b)).
 // ---------------------------------------------------------------------------
-template <int MODE, int PPL, bool CNT, bool POW2>
+template <int MODE, int PPL, bool CNT, bool POW2, bool MERGE>
 __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char* lds, uint32_t hbase,
                                          uint32_t ntpad4, const TaskCtx& t, int slab, int f2, int lane, int perm,
                                          double bp64, double bt64) {
@@ -221,6 +221,38 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
         const auto r2p = stream(p.dc + p2);
         const auto w2p = stream(p.we + p2);
         const auto d2p = stream(p.dw + p2);
+        // Sums of the products of lane pixel k with a block's forest-2 pixels, one set
+        // of LDS atomics per run of equal bins (consecutive forest-2 pixels of a lane
+        // often share a bin).
+        auto run_sums_impl = [&](auto UC, const uint32_t* bpk, const uint32_t* btk, double w1k,
+                                 const double* w2, double d1k, const double* d2) {
+            constexpr int UU = decltype(UC)::value;
+            uint32_t pk = 0xFFFFFFFFu, an = 0u;
+            double aw = 0.0, ad = 0.0;
+            auto flush = [&]() {
+                const uint32_t a4 = (pk << 2) + hbase;
+                if constexpr (CNT) lds_add<uint32_t>(lds, a4, an);
+                const uint32_t a8 = (a4 << 1) + woff;
+                lds_add<double>(lds, a8, aw);
+                lds_add<double>(lds, a8 + (L::DW - L::W), ad);
+            };
+#pragma unroll
+            for (int u = 0; u < UU; ++u) {
+                const uint32_t key = (bpk[u] < (uint32_t)NP && btk[u] < (uint32_t)NT)
+                                         ? __umul24(bpk[u], ntpad4 >> 2) + btk[u] : 0xFFFFFFFFu;
+                if (key != pk) {
+                    if (pk != 0xFFFFFFFFu) flush();
+                    aw = 0.0;
+                    ad = 0.0;
+                    an = 0u;
+                }
+                pk = key;
+                aw += w1k * w2[u];
+                ad += d1k * d2[u];
+                ++an;
+            }
+            if (pk != 0xFFFFFFFFu) flush();
+        };
         auto accumulate = [&](uint32_t bp, uint32_t bt, double pw, double pd) {
             if (bp < (uint32_t)NP && bt < (uint32_t)NT) {
                 if constexpr (CNT) {
@@ -248,6 +280,24 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                     w2[u] = w2p[jv + u];
                     d2[u] = d2p[jv + u];
                 }
+                if constexpr (MERGE) {
+#pragma unroll
+                for (int k = 0; k < PPL; ++k) {
+                    uint32_t bpk[UU], btk[UU];
+#pragma unroll
+                    for (int u = 0; u < UU; ++u) {
+                        double qp = fabs(r1[k] - r2[u]) * cq;
+                        double qt = (r1[k] + r2[u]) * sq;
+                        if constexpr (!POW2) {
+                            qp = qp * bp64;
+                            qt = qt * bt64;
+                        }
+                        bpk[u] = (uint32_t)__double2int_rz(qp);
+                        btk[u] = (uint32_t)__double2int_rz(qt);
+                    }
+                    run_sums_impl(UC, bpk, btk, w1[k], w2, d1[k], d2);
+                }
+                } else {
 #pragma unroll
                 for (int u = 0; u < UU; ++u) {
 #pragma unroll
@@ -262,8 +312,10 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                                    d1[k] * d2[u]);
                     }
                 }
+                }
             };
-            constexpr int U = LYA_UNROLL_FP64;
+            // merged runs grow with the block, so the merging kernel takes longer blocks
+            constexpr int U = MERGE ? 2 * LYA_UNROLL_FP64 : LYA_UNROLL_FP64;
             int j = jlo;
             for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
             for (; j < jhi; ++j) block(j, std::integral_constant<int, 1>{});
@@ -322,10 +374,23 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                         }
                     }
                 }
+                if constexpr (MERGE) {
 #pragma unroll
-                for (int u = 0; u < UU; ++u) {
+                    for (int k = 0; k < PPL; ++k) {
+                        uint32_t bpk[UU], btk[UU];
 #pragma unroll
-                    for (int k = 0; k < PPL; ++k) accumulate(bp[u][k], bt[u][k], w1[k] * w2[u], d1[k] * d2[u]);
+                        for (int u = 0; u < UU; ++u) {
+                            bpk[u] = bp[u][k];
+                            btk[u] = bt[u][k];
+                        }
+                        run_sums_impl(UC, bpk, btk, w1[k], w2, d1[k], d2);
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < UU; ++u) {
+#pragma unroll
+                        for (int k = 0; k < PPL; ++k) accumulate(bp[u][k], bt[u][k], w1[k] * w2[u], d1[k] * d2[u]);
+                    }
                 }
             };
             constexpr int U = LYA_UNROLL_MIXED;
@@ -481,7 +546,7 @@ __device__ void flush_tile(const lya_xi_problem& p, unsigned char* lds, int R, i
     }
 }
 
-template <int MODE, int PPL, int WAVES, bool CNT, bool POW2, int R>
+template <int MODE, int PPL, int WAVES, bool CNT, bool POW2, int R, bool MERGE>
 __global__ void __launch_bounds__(WAVES * 64) xi_kernel(const lya_xi_problem p, int ntpad) {
     using L = Layout<MODE>;
     // statically sized, so histogram addresses fold into the ds offset field
@@ -528,7 +593,7 @@ __global__ void __launch_bounds__(WAVES * 64) xi_kernel(const lya_xi_problem p, 
                 const int q = k / nslab;
                 const int slab = k - q * nslab;
                 const int f2 = cptr(p.nb_idx)[nb0 + q];
-                run_task<MODE, PPL, CNT, POW2>(p, lds, hbase, ntpad4, t, slab, f2, lane, perm, bp64, bt64);
+                run_task<MODE, PPL, CNT, POW2, MERGE>(p, lds, hbase, ntpad4, t, slab, f2, lane, perm, bp64, bt64);
             }
             if constexpr (MODE == LYA_MODE_MIXED_F32) {
                 // every wave done with the item; then the partials move to FP64 and are
@@ -1043,27 +1108,36 @@ __global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ d
 // ---------------------------------------------------------------------------
 using XiKernel = void (*)(const lya_xi_problem, int);
 
-template <int MODE, int PPL, bool CNT, bool POW2, int R>
+template <int MODE, int PPL, bool CNT, bool POW2, int R, bool MERGE>
 XiKernel pick_w(int waves) {
     if constexpr (R == 1) {
-        if (waves == 4) return xi_kernel<MODE, PPL, 4, CNT, POW2, 1>;
-        if (waves == 16) return xi_kernel<MODE, PPL, 16, CNT, POW2, 1>;
+        if (waves == 4) return xi_kernel<MODE, PPL, 4, CNT, POW2, 1, MERGE>;
+        if (waves == 16) return xi_kernel<MODE, PPL, 16, CNT, POW2, 1, MERGE>;
     }
-    if (waves == 8) return xi_kernel<MODE, PPL, 8, CNT, POW2, R>;
+    if (waves == 8) return xi_kernel<MODE, PPL, 8, CNT, POW2, R, MERGE>;
     return nullptr;
 }
-template <int MODE, bool CNT, bool POW2, int R>
+template <int MODE, bool CNT, bool POW2, int R, bool MERGE>
 XiKernel pick_p(int ppl, int waves) {
-    if (ppl == 1) return pick_w<MODE, 1, CNT, POW2, R>(waves);
-    if (ppl == 2) return pick_w<MODE, 2, CNT, POW2, R>(waves);
-    if (ppl == 4) return pick_w<MODE, 4, CNT, POW2, R>(waves);
+    if (ppl == 1) return pick_w<MODE, 1, CNT, POW2, R, MERGE>(waves);
+    if (ppl == 2) return pick_w<MODE, 2, CNT, POW2, R, MERGE>(waves);
+    if (ppl == 4) return pick_w<MODE, 4, CNT, POW2, R, MERGE>(waves);
+    return nullptr;
+}
+template <int MODE, bool POW2, bool MERGE>
+XiKernel pick_r(bool cnt, int R, int ppl, int waves) {
+    if (R == 1) return cnt ? pick_p<MODE, true, POW2, 1, MERGE>(ppl, waves) : pick_p<MODE, false, POW2, 1, MERGE>(ppl, waves);
+    if (R == 2) return cnt ? pick_p<MODE, true, POW2, 2, MERGE>(ppl, waves) : pick_p<MODE, false, POW2, 2, MERGE>(ppl, waves);
     return nullptr;
 }
 template <int MODE, bool POW2>
-XiKernel pick_c(bool cnt, int R, int ppl, int waves) {
-    if (R == 1) return cnt ? pick_p<MODE, true, POW2, 1>(ppl, waves) : pick_p<MODE, false, POW2, 1>(ppl, waves);
-    if (R == 2) return cnt ? pick_p<MODE, true, POW2, 2>(ppl, waves) : pick_p<MODE, false, POW2, 2>(ppl, waves);
-    return nullptr;
+XiKernel pick_c(bool cnt, int R, int ppl, int waves, bool merge) {
+    if constexpr (MODE == LYA_MODE_MIXED_F32) {
+        (void)merge;
+        return pick_r<MODE, POW2, false>(cnt, R, ppl, waves);
+    } else {
+        return merge ? pick_r<MODE, POW2, true>(cnt, R, ppl, waves) : pick_r<MODE, POW2, false>(cnt, R, ppl, waves);
+    }
 }
 
 template <bool CNT, bool POW2>
@@ -1097,6 +1171,7 @@ int resolve(const lya_xi_problem* p, const lya_xi_config* cfg, Resolved& r) {
     const bool cnt = !(cfg && cfg->no_counts);
     const bool pow2 = p && is_pow2((double)p->np_rp / p->rpmax) && is_pow2((double)p->np_rt / p->rtmax);
     const bool walk = cfg && cfg->walk;
+    const bool merge = cfg && cfg->merge;
     if (walk) {
         if (r.mode != LYA_MODE_FP64) return lya_fail(LYA_ERR_UNSUPPORTED, "walk 1 runs the FP64 mode only");
         if (p && p->max_len > WALK_MAX_LEN)
@@ -1107,12 +1182,12 @@ int resolve(const lya_xi_problem* p, const lya_xi_config* cfg, Resolved& r) {
         r.fn = pow2 ? (cnt ? pick_walk<true, true>(r.waves) : pick_walk<false, true>(r.waves))
                     : (cnt ? pick_walk<true, false>(r.waves) : pick_walk<false, false>(r.waves));
     } else if (r.mode == LYA_MODE_FP64)
-        r.fn = pow2 ? pick_c<LYA_MODE_FP64, true>(cnt, r.R, r.ppl, r.waves)
-                    : pick_c<LYA_MODE_FP64, false>(cnt, r.R, r.ppl, r.waves);
+        r.fn = pow2 ? pick_c<LYA_MODE_FP64, true>(cnt, r.R, r.ppl, r.waves, merge)
+                    : pick_c<LYA_MODE_FP64, false>(cnt, r.R, r.ppl, r.waves, merge);
     else if (r.mode == LYA_MODE_MIXED)
-        r.fn = pick_c<LYA_MODE_MIXED, false>(cnt, r.R, r.ppl, r.waves);
+        r.fn = pick_c<LYA_MODE_MIXED, false>(cnt, r.R, r.ppl, r.waves, merge);
     else
-        r.fn = pick_c<LYA_MODE_MIXED_F32, false>(cnt, r.R, r.ppl, r.waves);
+        r.fn = pick_c<LYA_MODE_MIXED_F32, false>(cnt, r.R, r.ppl, r.waves, merge);
     if (!r.fn) return lya_fail(LYA_ERR_ARG, "unsupported shape px_per_lane=%d waves=%d", r.ppl, r.waves);
     if (p) {
         if (p->np_rp < 1 || p->np_rt < 1 || !(p->rpmax > 0) || !(p->rtmax > 0))

@@ -151,6 +151,7 @@ def default_config(mode="mixed", **kw):
     cfg.no_counts = 0 if kw.get("counts", True) else 1
     cfg.lane_map = kw.get("lane_map", 1)
     cfg.walk = kw.get("walk", 0)
+    cfg.merge = kw.get("merge", 0)
     return cfg
 
 

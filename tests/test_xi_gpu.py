@@ -31,8 +31,8 @@ def _check(fs, off, idx, got, tol, label):
     assert np.all(np.abs(d - d0) <= tol * ad + 1e-300), label
 
 
-@pytest.mark.parametrize("mode", ["fp64", "mixed", "mixed32"])
-def test_xi_matches_oracle(small_set, small_pairs, mode):
+@pytest.mark.parametrize("mode,merge", [("fp64", 0), ("fp64", 1), ("mixed", 0), ("mixed", 1), ("mixed32", 0)])
+def test_xi_matches_oracle(small_set, small_pairs, mode, merge):
     """fp64 and mixed (exact FP64 products, FP64 LDS sums): 1e-12 of the absolute
     sums; mixed32 (FP32 products and FP32 LDS partials): 1e-5."""
     am, (off, idx) = small_pairs
@@ -43,8 +43,8 @@ def test_xi_matches_oracle(small_set, small_pairs, mode):
         fs.we = fs.we32.astype(np.float64)
         fs.dw = fs.dw32.astype(np.float64)
     eng = xi.XiEngine(fs, am, pairs=(off, idx))
-    eng.run(xi.default_config(mode))
-    _check(fs, off, idx, eng.results(), 1e-5 if mode == "mixed32" else 1e-12, mode)
+    eng.run(xi.default_config(mode, merge=merge))
+    _check(fs, off, idx, eng.results(), 1e-5 if mode == "mixed32" else 1e-12, "%s merge %d" % (mode, merge))
 
 
 def test_kernel_shapes(small_set, small_pairs):
@@ -52,23 +52,26 @@ def test_kernel_shapes(small_set, small_pairs):
     am, (off, idx) = small_pairs
     c0, _, _ = oracle.xi_gpu_form(small_set, off, idx)
     eng = xi.XiEngine(small_set, am, pairs=(off, idx))
-    for mode, ppl, (waves, rep), pad, lmap in itertools.product(["fp64", "mixed", "mixed32"], [1, 2, 4],
-                                                                 [(4, 1), (8, 1), (8, 2), (16, 1)], [0, 1],
-                                                                 [0, 1, 2]):
-        cfg = xi.default_config(mode, px_per_lane=ppl, waves=waves, replicas=rep, row_pad=pad, lane_map=lmap)
+    for mode, ppl, (waves, rep), pad, lmap, merge in itertools.product(["fp64", "mixed", "mixed32"], [1, 2, 4],
+                                                                        [(4, 1), (8, 1), (8, 2), (16, 1)], [0, 1],
+                                                                        [0, 1, 2], [0, 1]):
+        if merge and mode == "mixed32":
+            continue
+        cfg = xi.default_config(mode, px_per_lane=ppl, waves=waves, replicas=rep, row_pad=pad, lane_map=lmap,
+                                merge=merge)
         eng.run(cfg)
         c, _, _ = eng.results()
-        assert np.array_equal(c, c0), (mode, ppl, waves, rep, pad, lmap)
+        assert np.array_equal(c, c0), (mode, ppl, waves, rep, pad, lmap, merge)
 
 
 def test_without_counts(small_set, small_pairs):
     """Counts off: identical sums, out_cnt left zero."""
     am, (off, idx) = small_pairs
     eng = xi.XiEngine(small_set, am, pairs=(off, idx))
-    for mode in ("fp64", "mixed", "mixed32"):
-        eng.run(xi.default_config(mode))
+    for mode, merge in (("fp64", 0), ("fp64", 1), ("mixed", 0), ("mixed", 1), ("mixed32", 0)):
+        eng.run(xi.default_config(mode, merge=merge))
         _, w0, d0 = eng.results()
-        eng.run(xi.default_config(mode, counts=False))
+        eng.run(xi.default_config(mode, counts=False, merge=merge))
         c, w, d = eng.results()
         assert c.sum() == 0
         tol = 1e-5 if mode == "mixed32" else 1e-12
@@ -85,8 +88,10 @@ def test_adversarial_edges_bit_exact():
     c0, _, _ = oracle.xi_gpu_form(fs, off, idx)
     eng = xi.XiEngine(fs, am, pairs=(off, idx))
     for mode in ("mixed", "mixed32", "fp64"):
-        for ppl in (1, 2):
-            eng.run(xi.default_config(mode, px_per_lane=ppl))
+        for ppl, merge in ((1, 0), (2, 0), (1, 1), (2, 1)):
+            if merge and mode == "mixed32":
+                continue
+            eng.run(xi.default_config(mode, px_per_lane=ppl, merge=merge))
             c, _, _ = eng.results()
             assert np.array_equal(c, c0), (mode, ppl, int((c != c0).sum()))
 
