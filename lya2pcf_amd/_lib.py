@@ -47,8 +47,16 @@ class XiConfig(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+LYA_ERR_ARG, LYA_ERR_HIP, LYA_ERR_CAPACITY, LYA_ERR_UNSUPPORTED = -1, -2, -3, -4
+
+
 class LyaError(RuntimeError):
-    pass
+    """A nonzero LYA_ERR_* status; ``code`` holds it."""
+
+    def __init__(self, msg, code=0):
+        super().__init__(msg)
+        self.code = code
+
 
 
 _lib = None
@@ -91,7 +99,7 @@ def lib():
 
 def check(rc):
     if rc != 0:
-        raise LyaError("lya2pcf error %d: %s" % (rc, lib().lya_last_error().decode(errors="replace")))
+        raise LyaError("lya2pcf error %d: %s" % (rc, lib().lya_last_error().decode(errors="replace")), rc)
     return rc
 
 

@@ -12,6 +12,7 @@ forest_class.py:78-98.  Device buffers are torch tensors (plumbing only).
 """
 import ctypes as C
 import time
+import warnings
 
 import numpy as np
 
@@ -194,6 +195,16 @@ class XiEngine:
         self._items_host = None
         self.setup_seconds = {}
         clock = time.perf_counter
+        if self.pair_builder == "device":
+            try:
+                self._upload()
+                self._build_on_device(forests, angmax, f1_list, clock)
+            except _lib.LyaError as e:
+                if e.code != _lib.LYA_ERR_UNSUPPORTED:
+                    raise
+                # a forest with more neighbours than the device row sort holds
+                warnings.warn("device pair list unavailable (%s); using the host builder" % e)
+                self.pair_builder = "host"
         if self.pair_builder == "host":
             t0 = clock()
             nb_off, nb_idx = pairs if pairs is not None else build_pairs(forests, angmax)
@@ -206,23 +217,24 @@ class XiEngine:
             _, _, self.pixel_pairs = build_work(forests, nb_off, nb_idx, f1_list, self.item_cost, 1 << 62)
             self.setup_seconds = {"pairs": t1 - t0, "items": clock() - t1}
             self._upload()
-        else:
-            self._nb_host = None
-            self._upload()
-            d = self.d
-            torch.cuda.synchronize()
-            t0 = clock()
-            self.d_nb_off, nb_idx_t, self.n_pairs = build_pairs_device(d["ux"], d["uy"], d["uz"], d["ra"], d["name"],
-                                                                       angmax)
-            d["nb_idx"] = nb_idx_t if self.n_pairs else torch.zeros(1, dtype=torch.int32, device=self.device)
-            torch.cuda.synchronize()
-            t1 = clock()
-            f1_slot = np.ascontiguousarray(forests.slot[f1_list], dtype=np.int32)
-            items, costs, self.pixel_pairs = build_items_device(self._t(f1_list), self._t(f1_slot), d["pix_off"],
-                                                                self.d_nb_off, nb_idx_t, self.item_cost)
-            self._items_host = (items.cpu().numpy(), costs.cpu().numpy())
-            self.setup_seconds = {"pairs": t1 - t0, "items": clock() - t1}
         self._graphs = []
+
+    def _build_on_device(self, forests, angmax, f1_list, clock):
+        torch = self.torch
+        self._nb_host = None
+        d = self.d
+        torch.cuda.synchronize()
+        t0 = clock()
+        self.d_nb_off, nb_idx_t, self.n_pairs = build_pairs_device(d["ux"], d["uy"], d["uz"], d["ra"], d["name"],
+                                                                   angmax)
+        d["nb_idx"] = nb_idx_t if self.n_pairs else torch.zeros(1, dtype=torch.int32, device=self.device)
+        torch.cuda.synchronize()
+        t1 = clock()
+        f1_slot = np.ascontiguousarray(forests.slot[f1_list], dtype=np.int32)
+        items, costs, self.pixel_pairs = build_items_device(self._t(f1_list), self._t(f1_slot), d["pix_off"],
+                                                            self.d_nb_off, nb_idx_t, self.item_cost)
+        self._items_host = (items.cpu().numpy(), costs.cpu().numpy())
+        self.setup_seconds = {"pairs": t1 - t0, "items": clock() - t1}
 
     def _host_pairs(self):
         if self._nb_host is None:

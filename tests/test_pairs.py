@@ -173,3 +173,32 @@ def test_engine_device_builder_same_results(small_set, mode):
     assert np.array_equal(c0, c1)
     np.testing.assert_allclose(w1, w0, rtol=1e-12, atol=0)
     np.testing.assert_allclose(d1, d0, rtol=1e-9, atol=1e-12 * np.abs(w0).max())
+
+
+@pytest.mark.gpu
+def test_engine_falls_back_to_host_pair_list():
+    """A forest with more than 16384 neighbours: the engine warns and builds the list
+    on the host; the histograms still match the oracle."""
+    rng = np.random.default_rng(12)
+    n = 16500
+    ra = np.sort(rng.uniform(0, 1e-3, n))
+    dec = rng.uniform(-1e-3, 1e-3, n)
+    npx = 2
+    off = np.arange(n + 1, dtype=np.int64) * npx
+    dc = np.repeat(rng.uniform(3600.0, 3700.0, n), npx) + np.tile(np.arange(npx) * 2.0, n)
+    fs = forests.ForestSet(off, [0], ux=np.cos(dec) * np.cos(ra), uy=np.cos(dec) * np.sin(ra), uz=np.sin(dec),
+                           ra=ra, dec=dec, name=np.arange(n, dtype=np.int64), slot=np.zeros(n, np.int32),
+                           omega=np.ones(n), odl2=np.ones(n), dc=dc, we=rng.uniform(0.5, 2.0, n * npx),
+                           dw=rng.normal(size=n * npx), dl=np.zeros(n * npx))
+    am = forests.angmax_from(fs, 200.0)
+    print("fallback", n, flush=True)
+    with pytest.warns(UserWarning, match="host builder"):
+        eng = xi.XiEngine(fs, am)
+    assert eng.pair_builder == "host"
+    eng.run(xi.default_config("fp64"))
+    c, w, _ = eng.results()
+    off0, idx0 = oracle.pairs(fs, am)
+    assert np.array_equal(eng.nb_off, off0) and np.array_equal(eng.nb_idx, idx0)
+    c0, w0, _ = oracle.xi_gpu_form(fs, off0, idx0)
+    assert np.array_equal(c, c0)
+    np.testing.assert_allclose(w, w0, rtol=1e-10)
