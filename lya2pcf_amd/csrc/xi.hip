@@ -93,30 +93,6 @@ __device__ __forceinline__ uint32_t opaque(uint32_t x) {
     return x;
 }
 
-// Forest-2 pixels are wave-uniform.  With LYA_VMEM_J (default) they are read
-// with vector loads at a uniform address (the index is made opaque so the
-// compiler cannot turn them into scalar loads): vector loads complete on
-// vmcnt, while scalar loads share lgkmcnt with the LDS atomics, so every use of
-// a scalar-loaded value behind a skipped (exec-masked) accumulate would wait
-// for all atomics issued before it.  With vector loads the atomics are never
-// waited for inside the pair loop.
-#ifndef LYA_VMEM_J
-#define LYA_VMEM_J 0
-#endif
-template <class T>
-__device__ __forceinline__ auto stream(const T* base) {
-    if constexpr (LYA_VMEM_J)
-        return base;
-    else
-        return cptr(base);
-}
-__device__ __forceinline__ int jidx(int j) {
-    if constexpr (LYA_VMEM_J)
-        return (int)opaque((uint32_t)j);
-    else
-        return j;
-}
-
 template <class T>
 __device__ __forceinline__ void lds_add(unsigned char* lds, uint32_t byte_off, T v) {
     __hip_atomic_fetch_add((T*)(lds + byte_off), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -218,9 +194,10 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
             w1[k] = valid[k] ? p.we[pix1[k]] : 0.0;
             d1[k] = valid[k] ? p.dw[pix1[k]] : 0.0;
         }
-        const auto r2p = stream(p.dc + p2);
-        const auto w2p = stream(p.we + p2);
-        const auto d2p = stream(p.dw + p2);
+        // forest-2 pixels are wave-uniform: scalar loads through the constant cache
+        const CAS double* r2p = cptr(p.dc) + p2;
+        const CAS double* w2p = cptr(p.we) + p2;
+        const CAS double* d2p = cptr(p.dw) + p2;
         // Sums of the products of lane pixel k with a block's forest-2 pixels, one set
         // of LDS atomics per run of equal bins (consecutive forest-2 pixels of a lane
         // often share a bin).
@@ -273,7 +250,7 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
             auto block = [&](int j, auto UC) {
                 constexpr int UU = decltype(UC)::value;
                 double r2[UU], w2[UU], d2[UU];
-                const int jv = jidx(j);
+                const int jv = j;
 #pragma unroll
                 for (int u = 0; u < UU; ++u) {
                     r2[u] = r2p[jv + u];
@@ -334,7 +311,7 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                 P1[k] = valid[k] ? (float)((r1[k] - base2) * cb) : 1048576.5f;  // 2^20 + 1/2: out of range
                 T1[k] = valid[k] ? (float)((r1[k] + base2) * sb + (double)tau) : 1048576.5f;
             }
-            const auto o2p = stream(p.off32 + p2);
+            const CAS float* o2p = cptr(p.off32) + p2;
             auto block = [&](int j, auto UC) {
                 constexpr int UU = decltype(UC)::value;
                 uint32_t bp[UU][PPL], bt[UU][PPL];
@@ -342,7 +319,7 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                 float o2v[UU];
                 double w2[UU], d2[UU];
                 float frmin = 1.0f;
-                const int jv = jidx(j);
+                const int jv = j;
 #pragma unroll
                 for (int u = 0; u < UU; ++u) {
                     o2v[u] = o2p[jv + u];
@@ -422,10 +399,10 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                 d1[k] = 0.f;
             }
         }
-        const auto o2p = stream(p.off32 + p2);
-        const auto w2p = stream(p.we32 + p2);
-        const auto d2p = stream(p.dw32 + p2);
-        const auto r2p = stream(p.dc + p2);
+        const CAS float* o2p = cptr(p.off32) + p2;
+        const CAS float* w2p = cptr(p.we32) + p2;
+        const CAS float* d2p = cptr(p.dw32) + p2;
+        const CAS double* r2p = cptr(p.dc) + p2;
         constexpr int U = 8;
         auto block = [&](int j, auto UC) {
             constexpr int UU = decltype(UC)::value;
@@ -433,7 +410,7 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
             float fr[UU][PPL];
             float o2v[UU], w2v[UU], d2v[UU];
             float frmin = 1.0f;
-            const int jv = jidx(j);
+            const int jv = j;
 #pragma unroll
             for (int u = 0; u < UU; ++u) {
                 o2v[u] = o2p[jv + u];
