@@ -1,7 +1,9 @@
 """Consumer of the per-pixel histograms: mean xi, subsample covariance,
 smoothing and diagonal errors (reference post_processing.py:13-109, 122-169),
-vectorised with numpy (the reference uses numba loops; fitsio output is not
-written here, fitsio being unavailable).
+vectorised with numpy (the reference uses numba loops).  When
+``distortion.npy`` is present, the combined FITS table ``correlation.out.gz``
+(HDU ``COR``: DA, RP, RT, CO, DM and the binning/cosmology header, reference
+post_processing.py:186-216) is written with the package's FITS writer.
 
     python -m lya2pcf_amd.postproc [--corr-dir DIR] [--write-coordinates] [--diagonal-error]
 
@@ -88,12 +90,30 @@ def diag_error(da, we):
     return error, mda
 
 
+def write_fits(path, correlation, covariance, distortion, np_rp, np_rt, rpmax, rtmax, omega_m):
+    """The ``COR`` table of post_processing.py:186-216: one row per bin (r_p major)
+    with DA (xi), RP and RT (bin centres), CO (covariance row) and DM (distortion
+    row)."""
+    from . import fits
+    n = np_rp * np_rt
+    i, j = np.meshgrid(np.arange(np_rp), np.arange(np_rt), indexing="ij")
+    cols = {"DA": np.asarray(correlation, dtype=np.float64).ravel(),
+            "RP": ((i + 0.5) * rpmax / np_rp).ravel(), "RT": ((j + 0.5) * rtmax / np_rt).ravel(),
+            "CO": np.asarray(covariance, dtype=np.float64).reshape(n, n),
+            "DM": np.asarray(distortion, dtype=np.float64).reshape(n, n)}
+    header = {"RPMIN": 0, "RPMAX": float(rpmax), "RTMAX": float(rtmax), "NP": int(np_rp), "NT": int(np_rt),
+              "OMEGAM": float(omega_m), "OMEGAR": 0, "OMEGAK": 0, "WL": -1}
+    fits.write(path, [fits.table_bytes(cols, "COR", header)])
+
+
 def main(argv=None):
     try:
         import parameters as P
         corr_default, np_rp, np_rt, rpmax, rtmax = P.corr_dir, P.numpix_rp, P.numpix_rt, P.rpmax, P.rtmax
+        omega_m = float(P.Omm)
     except Exception:
         corr_default, np_rp, np_rt, rpmax, rtmax = "./outputs/test/", 50, 50, 200, 200
+        omega_m = 0.3153
     ap = argparse.ArgumentParser(description="Correlation and covariance from the histogram files.")
     ap.add_argument("--corr-dir", default=corr_default)
     ap.add_argument("--write-coordinates", action="store_true")
@@ -115,6 +135,16 @@ def main(argv=None):
         i, j = np.meshgrid(np.arange(np_rp), np.arange(np_rt), indexing="ij")
         np.save(os.path.join(a.corr_dir, "rp"), (i + 0.5) * rpmax / np_rp)
         np.save(os.path.join(a.corr_dir, "rt"), (j + 0.5) * rtmax / np_rt)
+    dpath = os.path.join(a.corr_dir, "distortion.npy")
+    if not os.path.exists(dpath):
+        warnings.warn("No distortion.npy found. Compute it first if you want to obtain a fits.gz file with all "
+                      "the outputs.")
+        return correlation, error
+    # with --diagonal-error the table carries the diagonal covariance error**2 (the
+    # reference has no covariance there and stops with a NameError)
+    cov_out = covariance if not a.diagonal_error else np.diag(error ** 2)
+    write_fits(os.path.join(a.corr_dir, "correlation.out.gz"), correlation, cov_out, np.load(dpath),
+               np_rp, np_rt, rpmax, rtmax, omega_m)
     return correlation, error
 
 

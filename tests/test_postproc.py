@@ -1,6 +1,7 @@
 """Post-processing consumer (reference post_processing.py:30-169) reading the
 histogram files the drop-in driver writes."""
 import numpy as np
+import pytest
 
 import oracle
 from lya2pcf_amd import postproc
@@ -65,3 +66,29 @@ def test_mean_xi_equals_summed_histograms(tmp_path, small_set, small_pairs):
     np.testing.assert_allclose(mean[ok], D[ok] / W[ok], rtol=1e-10, atol=1e-16)
     corr, err = postproc.main(["--corr-dir", str(tmp_path), "--diagonal-error"])
     assert corr.shape == (2500,) and (tmp_path / "correlation_2d.npy").exists()
+
+
+def test_fits_table_with_distortion(tmp_path, small_set, small_pairs):
+    """correlation.out.gz: the COR table of post_processing.py:186-216, read back with
+    the package's FITS reader (pinned separately by a hand-built FITS file)."""
+    from lya2pcf_amd import fits
+    am, (off, idx) = small_pairs
+    c, w, d = oracle.xi_gpu_form(small_set, off, idx)
+    for k, pix in enumerate(small_set.slots):
+        np.save(tmp_path / ("2d_histogram_pixel_%d" % pix), (w[k], d[k]))
+    dist = np.random.default_rng(3).normal(size=(2500, 2500))
+    np.save(tmp_path / "distortion", dist)
+    corr, err = postproc.main(["--corr-dir", str(tmp_path)])
+    hdus = fits.read(tmp_path / "correlation.out.gz")
+    t = fits.by_name(hdus, "COR")
+    assert t.header["NP"] == 50 and t.header["NT"] == 50 and t.header["RPMAX"] == 200.0 and t.header["WL"] == -1
+    assert np.array_equal(t.data["DA"], corr.ravel())
+    np.testing.assert_array_equal(t.data["DM"], dist)
+    np.testing.assert_array_equal(t.data["CO"], np.load(tmp_path / "covariance.npy"))
+    assert t.data["RP"][0] == 2.0 and t.data["RT"][1] == 6.0 and t.data["RP"][50] == 6.0
+    # without the distortion file: a warning and no FITS file
+    (tmp_path / "correlation.out.gz").unlink()
+    (tmp_path / "distortion.npy").unlink()
+    with pytest.warns(UserWarning, match="distortion"):
+        postproc.main(["--corr-dir", str(tmp_path), "--diagonal-error"])
+    assert not (tmp_path / "correlation.out.gz").exists()
