@@ -205,7 +205,11 @@ int lya_xi_adjoint(const lya_xi_problem* p, const lya_xi_config* cfg, const doub
  *   active in the pair, D[A][A] += w_i w_j, wA[A] += w_i w_j.
  * dl = delta_lambda, omega = sum w, odl2 = sum w dl^2 per forest (FP64).
  * D: [nb][nb] FP64 with nb = np_rp*np_rt, wA: [nb]; both accumulated.
+ * Each pair's contribution is symmetric, and only the upper triangle of D
+ * (row <= column) is accumulated: call lya_distortion_mirror() after the last
+ * lya_distortion_run() to copy it into the lower triangle.
  * scratch: device buffer of at least lya_distortion_scratch_bytes() bytes.
+ * Replaces the kernels launched at distortion_procedures_pycuda.py:249-279.
  */
 int64_t lya_distortion_scratch_bytes(int32_t groups, int32_t max_len, int32_t nbins);
 int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, const double* dc, const double* we,
@@ -214,6 +218,9 @@ int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, const double* d
                        int32_t max_len, int32_t np_rp, int32_t np_rt, double rpmax, double rtmax,
                        double* D, double* wA, void* scratch, int64_t scratch_bytes, int32_t groups,
                        void* stream);
+/* D[c][r] = D[r][c] for r < c: completes the symmetric D accumulated by
+ * lya_distortion_run (stream-ordered; idempotent). */
+int lya_distortion_mirror(double* D, int32_t nb, void* stream);
 
 /* Test helpers. */
 /* c = sqrt(0.5*(1+cos12)), s = sqrt(0.5*(1-cos12)) evaluated by the device code path. */

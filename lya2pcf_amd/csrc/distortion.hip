@@ -12,18 +12,28 @@
 //                        - dl_i eta31[j] - dl_j eta13[i] - eta21[j] - eta12[i])
 //   D[A][A] += w_i w_j,  wA[A] += w_i w_j.
 //
-// MI355X organisation (one workgroup per forest pair, persistent over pairs):
+// Summing the eight terms over the pairs of A first gives, per forest pair,
+//   D_pair[A][B] = sum_k Z[A][k] v[k] Z[B][k]  (+ S00[A] on the diagonal)
+// with one k per table row:
+//   k = j (n2 rows)  Z = eta21[.][j]  v = -w_j O1     k = j  Z = eta31[.][j]  v = -w_j L1
+//   k = i (n1 rows)  Z = eta12[.][i]  v = -w_i O2     k = i  Z = eta13[.][i]  v = -w_i L2
+//   4 rows  Z = S00, S01, S10, S11 (bin sums of w_i w_j, w_i w_j dl_j, w_i dl_i w_j,
+//           w_i dl_i w_j dl_j)  v = 1/(O1 O2), 1/(O1 L2), 1/(L1 O2), 1/(L1 L2).
+// D_pair is symmetric, a rank-K (K = 2 n1 + 2 n2 + 4) product over the nact active
+// bins: nact^2 K / 2 multiply-adds instead of n1 n2 nact histogram updates.
+//
+// MI355X organisation (one workgroup of 8 waves per forest pair, persistent over
+// an atomic queue of pairs):
 //   1. FP64 bins of all pixel pairs (the xi formula) into LDS (u16, j-major);
-//      active bins flagged and compacted (wave scan) to b = 0..nact-1.
-//   2. eta12/13 (one lane per i walking j) and eta21/31 (one lane per j walking
-//      i) accumulated in registers along runs of equal bin and written to a
-//      per-workgroup global scratch [nact][n]; eta22..33 reduced per b.
-//   3. For each active B: every pixel pair adds
-//        w_i w_j [ (K_j + dl_i L_j) + (M_i + dl_j N_i) ]
-//      with K_j = eta22 + dl_j eta23 - eta21[j], L_j = eta32 + dl_j eta33 - eta31[j],
-//      M_i = -eta12[i], N_i = -eta13[i], into an LDS histogram over compact A;
-//      the histogram is flushed once into column B of D (FP64 global atomics).
-//   The D update volume per pair is nact^2 instead of n1 n2 nact.
+//      active bins flagged and compacted (wave scan) to b = 0..nact-1 in bin order.
+//   2. Z is built in chunks of at most 64 rows in LDS, zs[kk][b] (LDS atomics of
+//      run sums: a thread walks 16 pixels of one row or column of the pair's bin
+//      table and adds one value per run of equal bin).
+//   3. Each wave holds up to TPW 32x32 tiles of the upper triangle of D_pair in
+//      registers (4x4 per lane) and adds every chunk's rank-kc update (FP64 fma).
+//   4. The tiles go into the upper triangle of D (FP64 global atomics, act is
+//      ascending so row <= column); lya_distortion_mirror() fills the lower
+//      triangle once accumulation is over.
 // Forests whose L = 0 (constant loglam) get zero slope terms (the reference
 // divides by zero there).
 #include <hip/hip_runtime.h>
@@ -44,6 +54,10 @@ constexpr int DWAVES = 8;
 constexpr int DTHREADS = DWAVES * 64;
 constexpr int MAX_PAIRS_LDS = 44000;   // u16 bins of one forest pair held in LDS
 constexpr uint16_t NOBIN = 0xFFFF;
+constexpr int TS = 32;       // D_pair tile edge; a lane holds 4 x 4 of a tile
+constexpr int TPW = 4;       // tiles per wave per pass over Z
+constexpr int KC_MAX = 64;   // Z rows per LDS chunk
+constexpr int SEG = 16;      // pixels walked by one thread while building a chunk
 
 template <class T>
 __device__ __forceinline__ const CAS T* cptr(const T* p) {
@@ -69,10 +83,14 @@ struct DArgs {
     double rpmax, rtmax;
     double* D;
     double* wA;
-    unsigned char* scratch;
-    int64_t per_group;   // scratch bytes per workgroup
+    int32_t zcap;        // doubles of the LDS chunk buffer
     uint32_t* queue;
 };
+
+// LDS bytes before the chunk buffer: bins, bin maps, chunk weights.
+__host__ __device__ constexpr size_t lds_fixed(int nb) {
+    return (((size_t)2 * MAX_PAIRS_LDS + 4 * (size_t)nb + 15) & ~(size_t)15) + 8 * KC_MAX;
+}
 
 __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -81,17 +99,10 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
     uint16_t* binc = (uint16_t*)lds;                                   // [MAX_PAIRS_LDS]
     int16_t* cmap = (int16_t*)(lds + 2 * MAX_PAIRS_LDS);               // [NB] bin -> compact, -1
     int16_t* act = cmap + NB;                                          // [NB] compact -> bin
-    double* hist = (double*)(lds + ((2 * MAX_PAIRS_LDS + 4 * NB + 15) & ~15));  // [NB]
+    double* vk = (double*)(lds + (((size_t)2 * MAX_PAIRS_LDS + 4 * NB + 15) & ~(size_t)15));  // [KC_MAX]
+    double* zs = (double*)(lds + lds_fixed(NB));                      // [zcap]: Z chunk [kk][b]
     const int tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
     const double bp64 = (double)a.np_rp / a.rpmax, bt64 = (double)a.np_rt / a.rtmax;
-
-    unsigned char* scr = a.scratch + (int64_t)blockIdx.x * a.per_group;
-    const int ML = a.max_len;
-    double* e12 = (double*)scr;                  // [NB][ML]
-    double* e13 = e12 + (int64_t)NB * ML;
-    double* e21 = e13 + (int64_t)NB * ML;
-    double* e31 = e21 + (int64_t)NB * ML;
-    double* e2x = e31 + (int64_t)NB * ML;        // [NB][4]: eta22, eta23, eta32, eta33
 
     for (;;) {
         if (tid == 0) s_pair = (int)atomicAdd(a.queue, 1u);
@@ -154,113 +165,186 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
             const uint16_t A = binc[k];
             binc[k] = (A == NOBIN) ? NOBIN : (uint16_t)cmap[A];
         }
-        // zero eta tables of this pair
-        for (int64_t k = tid; k < (int64_t)nact * ML; k += DTHREADS) {
-            e12[k] = 0.0; e13[k] = 0.0; e21[k] = 0.0; e31[k] = 0.0;
-        }
         __syncthreads();
+        if (nact == 0) continue;
 
-        // ---- 2. eta tables ---------------------------------------------------------
-        if (tid < n1) {
-            const int i = tid;
-            int cur = -1;
-            double s12 = 0.0, s13 = 0.0;
-            for (int j = 0; j < n2; ++j) {
-                const int b = binc[j * n1 + i];
-                if (b == NOBIN) continue;
-                if (b != cur) {
-                    if (cur >= 0) { e12[(int64_t)cur * ML + i] += s12; e13[(int64_t)cur * ML + i] += s13; }
-                    cur = b; s12 = 0.0; s13 = 0.0;
-                }
-                const double wj = cptr(a.we)[p2 + j];
-                s12 += wj * iO2;
-                s13 += wj * cptr(a.dl)[p2 + j] * iL2;
+        // ---- 2. D_pair = Z diag(v) Z^T over the upper triangle ----------------------
+        const int T = (nact + TS - 1) / TS;
+        const int npad = T * TS;
+        const int ntiles = T * (T + 1) / 2;
+        const int kc_max = min(KC_MAX, (a.zcap / npad) & ~1);
+        const int nseg1 = (n1 + SEG - 1) / SEG, nseg2 = (n2 + SEG - 1) / SEG;
+        const int nchunk_j = (2 * n2 + kc_max - 1) / kc_max, nchunk_i = (2 * n1 + kc_max - 1) / kc_max;
+        const int nchunk = nchunk_j + nchunk_i + (kc_max >= 4 ? 1 : 2);
+        for (int t0 = 0; t0 < ntiles; t0 += DWAVES * TPW) {
+            // this wave's tiles: t0 + wave*TPW + q, as (row tile, column tile >= row tile)
+            int tra[TPW], trb[TPW];
+            bool tv[TPW];
+#pragma unroll
+            for (int q = 0; q < TPW; ++q) {
+                int t = t0 + wave * TPW + q;
+                tv[q] = t < ntiles;
+                int ta = 0;
+                while (tv[q] && t >= T - ta) { t -= T - ta; ++ta; }
+                tra[q] = ta * TS + (lane >> 3) * 4;
+                trb[q] = (ta + t) * TS + (lane & 7) * 4;
             }
-            if (cur >= 0) { e12[(int64_t)cur * ML + i] += s12; e13[(int64_t)cur * ML + i] += s13; }
-        } else if (tid >= 256 && tid - 256 < n2) {
-            const int j = tid - 256;
-            int cur = -1;
-            double s21 = 0.0, s31 = 0.0;
-            for (int i = 0; i < n1; ++i) {
-                const int b = binc[j * n1 + i];
-                if (b == NOBIN) continue;
-                if (b != cur) {
-                    if (cur >= 0) { e21[(int64_t)cur * ML + j] += s21; e31[(int64_t)cur * ML + j] += s31; }
-                    cur = b; s21 = 0.0; s31 = 0.0;
-                }
-                const double wi = cptr(a.we)[p1 + i];
-                s21 += wi * iO1;
-                s31 += wi * cptr(a.dl)[p1 + i] * iL1;
-            }
-            if (cur >= 0) { e21[(int64_t)cur * ML + j] += s21; e31[(int64_t)cur * ML + j] += s31; }
-        }
-        __syncthreads();
-        // eta22..33 per active bin; identity term and weights
-        for (int b = tid; b < nact; b += DTHREADS) {
-            double t22 = 0.0, t23 = 0.0, t32 = 0.0, t33 = 0.0;
-            for (int i = 0; i < n1; ++i) {
-                const double wi = cptr(a.we)[p1 + i];
-                const double a1 = wi * iO1, b1 = wi * cptr(a.dl)[p1 + i] * iL1;
-                const double x12 = e12[(int64_t)b * ML + i], x13 = e13[(int64_t)b * ML + i];
-                t22 += a1 * x12; t23 += a1 * x13; t32 += b1 * x12; t33 += b1 * x13;
-            }
-            e2x[4 * b + 0] = t22; e2x[4 * b + 1] = t23; e2x[4 * b + 2] = t32; e2x[4 * b + 3] = t33;
-        }
-        __syncthreads();
-        // identity D[A][A] and wA[A]: sum over pairs in A of w_i w_j
-        for (int b = tid; b < nact; b += DTHREADS) hist[b] = 0.0;
-        __syncthreads();
-        for (int k = tid; k < npp; k += DTHREADS) {
-            const uint16_t b = binc[k];
-            if (b == NOBIN) continue;
-            const int j = k / n1, i = k - j * n1;
-            atomicAdd(&hist[b], cptr(a.we)[p1 + i] * a.we[p2 + j]);
-        }
-        __syncthreads();
-        for (int b = tid; b < nact; b += DTHREADS) {
-            const int A = act[b];
-            unsafeAtomicAdd(a.wA + A, hist[b]);
-            unsafeAtomicAdd(a.D + (int64_t)A * NB + A, hist[b]);
-        }
-        __syncthreads();
+            double acc[TPW][16];
+#pragma unroll
+            for (int q = 0; q < TPW; ++q)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) acc[q][e] = 0.0;
 
-        // ---- 3. per active B: histogram over A, flushed into column B -------------
-        const int nslab = (n1 + 63) / 64;
-        const int slab = wave % nslab;
-        const int kw = wave / nslab;
-        const int mw = (DWAVES - slab + nslab - 1) / nslab;  // waves on this slab
-        const bool haswork = kw < mw;
-        const int i = slab * 64 + lane;
-        const bool vi = haswork && i < n1;
-        const double wi = vi ? a.we[p1 + i] : 0.0;
-        const double dli = vi ? a.dl[p1 + i] : 0.0;
-        for (int b = 0; b < nact; ++b) {
-            for (int q = tid; q < nact; q += DTHREADS) hist[q] = 0.0;
-            __syncthreads();
-            const double h22 = e2x[4 * b + 0], h23 = e2x[4 * b + 1], h32 = e2x[4 * b + 2], h33 = e2x[4 * b + 3];
-            if (haswork) {
-                const double Mi = vi ? -e12[(int64_t)b * ML + i] : 0.0;
-                const double Ni = vi ? -e13[(int64_t)b * ML + i] : 0.0;
-                for (int j = kw; j < n2; j += mw) {
-                    const double wj = cptr(a.we)[p2 + j], dlj = cptr(a.dl)[p2 + j];
-                    const double Kj = (h22 + dlj * h23) - e21[(int64_t)b * ML + j];
-                    const double Lj = (h32 + dlj * h33) - e31[(int64_t)b * ML + j];
-                    if (vi) {
-                        const uint16_t A = binc[j * n1 + i];
-                        if (A != NOBIN) {
-                            const double t = (Kj + dli * Lj) + (Mi + dlj * Ni);
-                            atomicAdd(&hist[A], (wi * wj) * t);
+            for (int ch = 0; ch < nchunk; ++ch) {
+                // build the chunk's rows of Z in zs[kk][a] and their weights in vk[kk]
+                int kc;
+                if (ch < nchunk_j) {
+                    // rows (e21, e31) of forest-2 pixels j0 .. j0 + kc/2
+                    const int j0 = ch * (kc_max / 2);
+                    const int nj = min(kc_max / 2, n2 - j0);
+                    kc = 2 * nj;
+                    for (int e = tid; e < kc * npad; e += DTHREADS) zs[e] = 0.0;
+                    if (tid < kc) {
+                        const double wj = cptr(a.we)[p2 + j0 + (tid >> 1)];
+                        vk[tid] = (tid & 1) ? -wj * L1 : -wj * O1;
+                    }
+                    __syncthreads();
+                    for (int it = tid; it < nj * nseg1; it += DTHREADS) {
+                        const int jj = it / nseg1, i0 = (it - jj * nseg1) * SEG, i1 = min(n1, i0 + SEG);
+                        const uint16_t* row = binc + (j0 + jj) * n1;
+                        double* z0 = zs + (2 * jj) * npad;
+                        int cur = -1;
+                        double s0 = 0.0, s1 = 0.0;
+                        for (int i = i0; i < i1; ++i) {
+                            const int b = row[i];
+                            if (b == NOBIN) continue;
+                            if (b != cur) {
+                                if (cur >= 0) { atomicAdd(z0 + cur, s0); atomicAdd(z0 + npad + cur, s1); }
+                                cur = b; s0 = 0.0; s1 = 0.0;
+                            }
+                            const double wi = cptr(a.we)[p1 + i];
+                            s0 += wi * iO1;
+                            s1 += wi * cptr(a.dl)[p1 + i] * iL1;
                         }
+                        if (cur >= 0) { atomicAdd(z0 + cur, s0); atomicAdd(z0 + npad + cur, s1); }
+                    }
+                } else if (ch < nchunk_j + nchunk_i) {
+                    // rows (e12, e13) of forest-1 pixels i0 .. i0 + kc/2
+                    const int i0 = (ch - nchunk_j) * (kc_max / 2);
+                    const int ni = min(kc_max / 2, n1 - i0);
+                    kc = 2 * ni;
+                    for (int e = tid; e < kc * npad; e += DTHREADS) zs[e] = 0.0;
+                    if (tid < kc) {
+                        const double wi = cptr(a.we)[p1 + i0 + (tid >> 1)];
+                        vk[tid] = (tid & 1) ? -wi * L2 : -wi * O2;
+                    }
+                    __syncthreads();
+                    for (int it = tid; it < ni * nseg2; it += DTHREADS) {
+                        const int ii = it / nseg2, j0 = (it - ii * nseg2) * SEG, j1 = min(n2, j0 + SEG);
+                        const uint16_t* col = binc + i0 + ii;
+                        double* z0 = zs + (2 * ii) * npad;
+                        int cur = -1;
+                        double s0 = 0.0, s1 = 0.0;
+                        for (int j = j0; j < j1; ++j) {
+                            const int b = col[j * n1];
+                            if (b == NOBIN) continue;
+                            if (b != cur) {
+                                if (cur >= 0) { atomicAdd(z0 + cur, s0); atomicAdd(z0 + npad + cur, s1); }
+                                cur = b; s0 = 0.0; s1 = 0.0;
+                            }
+                            const double wj = cptr(a.we)[p2 + j];
+                            s0 += wj * iO2;
+                            s1 += wj * cptr(a.dl)[p2 + j] * iL2;
+                        }
+                        if (cur >= 0) { atomicAdd(z0 + cur, s0); atomicAdd(z0 + npad + cur, s1); }
+                    }
+                } else {
+                    // rank rows r0 .. r0 + kc of S00, S01, S10, S11 = sums over the pixel
+                    // pairs of a bin of w_i w_j, w_i w_j dl_j, w_i dl_i w_j, w_i dl_i w_j dl_j
+                    // (one chunk, or two of two rows when the LDS holds only two rows)
+                    const int r0 = (ch - nchunk_j - nchunk_i) * 2 * (kc_max >= 4 ? 2 : 1);
+                    kc = kc_max >= 4 ? 4 : 2;
+                    for (int e = tid; e < kc * npad; e += DTHREADS) zs[e] = 0.0;
+                    if (tid < kc) {
+                        const int q = r0 + tid;
+                        vk[tid] = (q < 2 ? iO1 : iL1) * ((q & 1) ? iL2 : iO2);
+                    }
+                    __syncthreads();
+                    for (int it = tid; it < n2 * nseg1; it += DTHREADS) {
+                        const int j = it / nseg1, i0 = (it - j * nseg1) * SEG, i1 = min(n1, i0 + SEG);
+                        const uint16_t* row = binc + j * n1;
+                        const double wj = cptr(a.we)[p2 + j], wdj = wj * cptr(a.dl)[p2 + j];
+                        int cur = -1;
+                        double s0 = 0.0, s1 = 0.0;   // sum w_i, sum w_i dl_i over the run
+                        auto flush = [&]() {
+                            if (r0 == 0) {
+                                atomicAdd(zs + cur, s0 * wj);
+                                atomicAdd(zs + npad + cur, s0 * wdj);
+                            }
+                            if (r0 + kc > 2) {
+                                atomicAdd(zs + (2 - r0) * npad + cur, s1 * wj);
+                                atomicAdd(zs + (3 - r0) * npad + cur, s1 * wdj);
+                            }
+                        };
+                        for (int i = i0; i < i1; ++i) {
+                            const int b = row[i];
+                            if (b == NOBIN) continue;
+                            if (b != cur) {
+                                if (cur >= 0) flush();
+                                cur = b; s0 = 0.0; s1 = 0.0;
+                            }
+                            const double wi = cptr(a.we)[p1 + i];
+                            s0 += wi;
+                            s1 += wi * cptr(a.dl)[p1 + i];
+                        }
+                        if (cur >= 0) flush();
+                    }
+                }
+                __syncthreads();
+                if (ch == nchunk_j + nchunk_i && t0 == 0) {   // the chunk holding S00
+                    // identity term and weights: D[A][A] += S00[A], wA[A] += S00[A]
+                    for (int b = tid; b < nact; b += DTHREADS) {
+                        const int A = act[b];
+                        unsafeAtomicAdd(a.wA + A, zs[b]);
+                        unsafeAtomicAdd(a.D + (int64_t)A * NB + A, zs[b]);
+                    }
+                }
+                // rank-kc update of this wave's tiles
+                for (int kk = 0; kk < kc; ++kk) {
+                    const double v = vk[kk];
+                    const double* zr = zs + kk * npad;
+#pragma unroll
+                    for (int q = 0; q < TPW; ++q) {
+                        if (!tv[q]) continue;
+                        const double2 a01 = *(const double2*)(zr + tra[q]);
+                        const double2 a23 = *(const double2*)(zr + tra[q] + 2);
+                        const double2 b01 = *(const double2*)(zr + trb[q]);
+                        const double2 b23 = *(const double2*)(zr + trb[q] + 2);
+                        const double av[4] = {a01.x * v, a01.y * v, a23.x * v, a23.y * v};
+                        const double bv[4] = {b01.x, b01.y, b23.x, b23.y};
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+#pragma unroll
+                            for (int c2 = 0; c2 < 4; ++c2) acc[q][4 * r + c2] = fma(av[r], bv[c2], acc[q][4 * r + c2]);
+                    }
+                }
+                __syncthreads();
+            }
+            // upper triangle into D (act is ascending, so act[ra] <= act[rb] when ra <= rb)
+#pragma unroll
+            for (int q = 0; q < TPW; ++q) {
+                if (!tv[q]) continue;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int ra = tra[q] + r;
+#pragma unroll
+                    for (int c2 = 0; c2 < 4; ++c2) {
+                        const int rb = trb[q] + c2;
+                        if (ra <= rb && rb < nact)
+                            unsafeAtomicAdd(a.D + (int64_t)act[ra] * NB + act[rb], acc[q][4 * r + c2]);
                     }
                 }
             }
-            __syncthreads();
-            const int B = act[b];
-            for (int q = tid; q < nact; q += DTHREADS) {
-                const double v = hist[q];
-                if (v != 0.0) unsafeAtomicAdd(a.D + (int64_t)act[q] * NB + B, v);
-            }
-            __syncthreads();
         }
     }
 }
@@ -269,15 +353,19 @@ __global__ void zero_queue(uint32_t* q) {
     if (threadIdx.x < 4) q[threadIdx.x] = 0u;
 }
 
-int64_t per_group_bytes(int32_t max_len, int32_t nbins) {
-    return ((int64_t)4 * nbins * max_len + 4LL * nbins) * 8 + 256;
+// D[c][r] = D[r][c] for r < c (one thread per element of the lower triangle's rows).
+__global__ void mirror_kernel(double* D, int nb) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)nb * nb) return;
+    const int r = (int)(k / nb), c = (int)(k - (int64_t)r * nb);
+    if (c < r) D[k] = D[(int64_t)c * nb + r];
 }
 
 }  // namespace
 
 extern "C" int64_t lya_distortion_scratch_bytes(int32_t groups, int32_t max_len, int32_t nbins) {
     if (groups < 1 || max_len < 1 || nbins < 1) return -1;
-    return (int64_t)groups * per_group_bytes(max_len, nbins) + 64;
+    return 64;
 }
 
 extern "C" int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, const double* dc, const double* we,
@@ -296,30 +384,47 @@ extern "C" int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, cons
     if (max_len < 1 || (int64_t)max_len * max_len > MAX_PAIRS_LDS)
         return lya_fail(LYA_ERR_UNSUPPORTED, "distortion: max_len^2 = %lld exceeds %d pixel pairs per forest pair",
                         (long long)max_len * max_len, MAX_PAIRS_LDS);
-    if (max_len > 256) return lya_fail(LYA_ERR_UNSUPPORTED, "distortion: forests longer than 256 pixels");
     if (groups < 1) return lya_fail(LYA_ERR_ARG, "distortion: groups < 1");
     const int64_t need = lya_distortion_scratch_bytes(groups, max_len, NB);
     if (scratch_bytes < need)
         return lya_fail(LYA_ERR_CAPACITY, "distortion: scratch %lld < %lld bytes", (long long)scratch_bytes,
                         (long long)need);
-    const size_t lds = ((2 * MAX_PAIRS_LDS + 4 * NB + 15) & ~15) + (size_t)8 * NB;
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
         return lya_fail(LYA_ERR_HIP, "no HIP device");
-    if (lds > prop.sharedMemPerBlock) return lya_fail(LYA_ERR_UNSUPPORTED, "distortion: LDS %zu too large", lds);
-    (void)hipFuncSetAttribute((const void*)distortion_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    // the chunk buffer takes the rest of the workgroup's LDS; it must hold two rows of
+    // every active bin (padded to the tile edge)
+    const size_t fixed = lds_fixed(NB) + 256;   // 256: the kernel's static LDS
+    const int64_t zcap = fixed < prop.sharedMemPerBlock ? (int64_t)((prop.sharedMemPerBlock - fixed) / 8) : 0;
+    const int64_t npad_max = (int64_t)(NB + TS - 1) / TS * TS;
+    if (zcap < 2 * npad_max)
+        return lya_fail(LYA_ERR_UNSUPPORTED, "distortion: %d bins leave %lld doubles of LDS for Z chunks (< %lld)",
+                        NB, (long long)zcap, (long long)(2 * npad_max));
+    const int32_t zc = (int32_t)std::min<int64_t>(zcap, (int64_t)KC_MAX * npad_max);
+    const size_t lds = lds_fixed(NB) + (size_t)8 * zc;
+    if (hipFuncSetAttribute((const void*)distortion_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+        hipSuccess)
+        return lya_fail(LYA_ERR_HIP, "distortion: cannot reserve %zu bytes of LDS", lds);
     DArgs a;
     a.n_forest = n_forest; a.pix_off = pix_off; a.dc = dc; a.we = we; a.dl = dl; a.ux = ux; a.uy = uy; a.uz = uz;
     a.omega = omega; a.odl2 = odl2; a.pairs = pairs; a.n_pairs = n_pairs; a.max_len = max_len;
     a.np_rp = np_rp; a.np_rt = np_rt; a.rpmax = rpmax; a.rtmax = rtmax; a.D = D; a.wA = wA;
-    a.per_group = per_group_bytes(max_len, NB);
-    a.queue = (uint32_t*)((unsigned char*)scratch + (int64_t)groups * a.per_group);
-    a.scratch = (unsigned char*)scratch;
+    a.zcap = zc;
+    a.queue = (uint32_t*)scratch;
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(zero_queue, dim3(1), dim3(64), 0, st, a.queue);
     if (n_pairs > 0) hipLaunchKernelGGL(distortion_kernel, dim3(groups), dim3(DTHREADS), lds, st, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return lya_fail(LYA_ERR_HIP, "distortion launch: %s", hipGetErrorString(e));
+    return LYA_OK;
+}
+
+extern "C" int lya_distortion_mirror(double* D, int32_t nb, void* stream) {
+    if (!D || nb < 1) return lya_fail(LYA_ERR_ARG, "distortion mirror: bad arguments");
+    const int64_t n = (int64_t)nb * nb;
+    hipLaunchKernelGGL(mirror_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, D, nb);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return lya_fail(LYA_ERR_HIP, "distortion mirror launch: %s", hipGetErrorString(e));
     return LYA_OK;
 }

@@ -93,6 +93,8 @@ class DistortionEngine:
         self._keep = pt
 
     def result(self):
+        """(D, wA) on the host; D is completed from its accumulated upper triangle."""
+        check(lib().lya_distortion_mirror(ptr(self.D), self.nb, _lib.stream_ptr(None)))
         self.torch.cuda.synchronize()
         return self.D.cpu().numpy(), self.wA.cpu().numpy()
 
