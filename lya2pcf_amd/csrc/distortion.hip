@@ -29,8 +29,9 @@
 //   2. Z is built in chunks of at most 64 rows in LDS, zs[kk][b] (LDS atomics of
 //      run sums: a thread walks 16 pixels of one row or column of the pair's bin
 //      table and adds one value per run of equal bin).
-//   3. Each wave holds up to TPW 32x32 tiles of the upper triangle of D_pair in
-//      registers (4x4 per lane) and adds every chunk's rank-kc update (FP64 fma).
+//   3. The upper triangle of D_pair is cut into 16x16 blocks dealt round-robin to
+//      the waves; each block is one v_mfma_f64_16x16x4f64 accumulator (FP64 matrix
+//      cores, four Z rows per instruction), updated by every chunk.
 //   4. The tiles go into the upper triangle of D (FP64 global atomics, act is
 //      ascending so row <= column); lya_distortion_mirror() fills the lower
 //      triangle once accumulation is over.
@@ -50,12 +51,14 @@
 
 namespace {
 
+typedef double double4v __attribute__((ext_vector_type(4)));
+
 constexpr int DWAVES = 8;
 constexpr int DTHREADS = DWAVES * 64;
 constexpr int MAX_PAIRS_LDS = 44000;   // u16 bins of one forest pair held in LDS
 constexpr uint16_t NOBIN = 0xFFFF;
-constexpr int TS = 32;       // D_pair tile edge; a lane holds 4 x 4 of a tile
-constexpr int TPW = 4;       // tiles per wave per pass over Z
+constexpr int BS = 16;       // D_pair block edge: one v_mfma_f64_16x16x4f64 accumulator
+constexpr int BPW = 16;      // blocks per wave per pass over Z
 constexpr int KC_MAX = 64;   // Z rows per LDS chunk
 constexpr int SEG = 16;      // pixels walked by one thread while building a chunk
 
@@ -169,31 +172,30 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
         if (nact == 0) continue;
 
         // ---- 2. D_pair = Z diag(v) Z^T over the upper triangle ----------------------
-        const int T = (nact + TS - 1) / TS;
-        const int npad = T * TS;
-        const int ntiles = T * (T + 1) / 2;
+        const int NBK = (nact + BS - 1) / BS;
+        const int npad = NBK * BS;
+        const int nblk = NBK * (NBK + 1) / 2;
         const int kc_max = min(KC_MAX, (a.zcap / npad) & ~1);
         const int nseg1 = (n1 + SEG - 1) / SEG, nseg2 = (n2 + SEG - 1) / SEG;
         const int nchunk_j = (2 * n2 + kc_max - 1) / kc_max, nchunk_i = (2 * n1 + kc_max - 1) / kc_max;
         const int nchunk = nchunk_j + nchunk_i + (kc_max >= 4 ? 1 : 2);
-        for (int t0 = 0; t0 < ntiles; t0 += DWAVES * TPW) {
-            // this wave's tiles: t0 + wave*TPW + q, as (row tile, column tile >= row tile)
-            int tra[TPW], trb[TPW];
-            bool tv[TPW];
+        for (int t0 = 0; t0 < nblk; t0 += DWAVES * BPW) {
+            // this wave's 16x16 blocks t0 + q*DWAVES + wave, as (row block, column block
+            // >= row block); the lane's A and B operand column within the block
+            int boa[BPW], bob[BPW];
+            bool bv[BPW];
 #pragma unroll
-            for (int q = 0; q < TPW; ++q) {
-                int t = t0 + wave * TPW + q;
-                tv[q] = t < ntiles;
-                int ta = 0;
-                while (tv[q] && t >= T - ta) { t -= T - ta; ++ta; }
-                tra[q] = ta * TS + (lane >> 3) * 4;
-                trb[q] = (ta + t) * TS + (lane & 7) * 4;
+            for (int q = 0; q < BPW; ++q) {
+                int t = t0 + q * DWAVES + wave;
+                bv[q] = t < nblk;
+                int ba = 0;
+                while (bv[q] && t >= NBK - ba) { t -= NBK - ba; ++ba; }
+                boa[q] = ba * BS + (lane & 15);
+                bob[q] = (ba + t) * BS + (lane & 15);
             }
-            double acc[TPW][16];
+            double4v acc[BPW];
 #pragma unroll
-            for (int q = 0; q < TPW; ++q)
-#pragma unroll
-                for (int e = 0; e < 16; ++e) acc[q][e] = 0.0;
+            for (int q = 0; q < BPW; ++q) acc[q] = (double4v){0.0, 0.0, 0.0, 0.0};
 
             for (int ch = 0; ch < nchunk; ++ch) {
                 // build the chunk's rows of Z in zs[kk][a] and their weights in vk[kk]
@@ -309,40 +311,32 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
                         unsafeAtomicAdd(a.D + (int64_t)A * NB + A, zs[b]);
                     }
                 }
-                // rank-kc update of this wave's tiles
-                for (int kk = 0; kk < kc; ++kk) {
-                    const double v = vk[kk];
-                    const double* zr = zs + kk * npad;
+                // rank-kc update of this wave's blocks: lane l supplies A[l%16][k] = Z[row][k] v[k]
+                // and B[k][l%16] = Z[col][k] for k = kk0 + l/16 (A = 0 past the chunk)
+                for (int kk0 = 0; kk0 < kc; kk0 += 4) {
+                    const int k = kk0 + (lane >> 4);
+                    const bool kv = k < kc;
+                    const double v = kv ? vk[k] : 0.0;
+                    const double* zr = zs + (kv ? k : 0) * npad;
 #pragma unroll
-                    for (int q = 0; q < TPW; ++q) {
-                        if (!tv[q]) continue;
-                        const double2 a01 = *(const double2*)(zr + tra[q]);
-                        const double2 a23 = *(const double2*)(zr + tra[q] + 2);
-                        const double2 b01 = *(const double2*)(zr + trb[q]);
-                        const double2 b23 = *(const double2*)(zr + trb[q] + 2);
-                        const double av[4] = {a01.x * v, a01.y * v, a23.x * v, a23.y * v};
-                        const double bv[4] = {b01.x, b01.y, b23.x, b23.y};
-#pragma unroll
-                        for (int r = 0; r < 4; ++r)
-#pragma unroll
-                            for (int c2 = 0; c2 < 4; ++c2) acc[q][4 * r + c2] = fma(av[r], bv[c2], acc[q][4 * r + c2]);
+                    for (int q = 0; q < BPW; ++q) {
+                        if (!bv[q]) continue;
+                        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(zr[boa[q]] * v, zr[bob[q]], acc[q], 0, 0, 0);
                     }
                 }
                 __syncthreads();
             }
-            // upper triangle into D (act is ascending, so act[ra] <= act[rb] when ra <= rb)
+            // upper triangle into D (act is ascending, so act[row] <= act[col] when row <= col);
+            // accumulator element r of lane l is D_pair[l/16 + 4 r][l%16] of the block
 #pragma unroll
-            for (int q = 0; q < TPW; ++q) {
-                if (!tv[q]) continue;
+            for (int q = 0; q < BPW; ++q) {
+                if (!bv[q]) continue;
+                const int col = bob[q];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int ra = tra[q] + r;
-#pragma unroll
-                    for (int c2 = 0; c2 < 4; ++c2) {
-                        const int rb = trb[q] + c2;
-                        if (ra <= rb && rb < nact)
-                            unsafeAtomicAdd(a.D + (int64_t)act[ra] * NB + act[rb], acc[q][4 * r + c2]);
-                    }
+                    const int row = boa[q] - (lane & 15) + (lane >> 4) + 4 * r;
+                    if (row <= col && col < nact)
+                        unsafeAtomicAdd(a.D + (int64_t)act[row] * NB + act[col], acc[q][r]);
                 }
             }
         }
@@ -397,7 +391,7 @@ extern "C" int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, cons
     // every active bin (padded to the tile edge)
     const size_t fixed = lds_fixed(NB) + 256;   // 256: the kernel's static LDS
     const int64_t zcap = fixed < prop.sharedMemPerBlock ? (int64_t)((prop.sharedMemPerBlock - fixed) / 8) : 0;
-    const int64_t npad_max = (int64_t)(NB + TS - 1) / TS * TS;
+    const int64_t npad_max = (int64_t)(NB + BS - 1) / BS * BS;
     if (zcap < 2 * npad_max)
         return lya_fail(LYA_ERR_UNSUPPORTED, "distortion: %d bins leave %lld doubles of LDS for Z chunks (< %lld)",
                         NB, (long long)zcap, (long long)(2 * npad_max));
