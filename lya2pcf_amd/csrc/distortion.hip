@@ -26,9 +26,9 @@
 // an atomic queue of pairs):
 //   1. FP64 bins of all pixel pairs (the xi formula) into LDS (u16, j-major);
 //      active bins flagged and compacted (wave scan) to b = 0..nact-1 in bin order.
-//   2. Z is built in chunks of at most 64 rows in LDS, zs[kk][b] (LDS atomics of
-//      run sums: a thread walks 16 pixels of one row or column of the pair's bin
-//      table and adds one value per run of equal bin).
+//   2. Z is built in chunks of at most 64 rows in LDS, zs[kk][b]: one thread per
+//      pixel pair of the chunk adds its weight into its bin (LDS atomics); the
+//      pixel weights w and w dl of both forests are staged in LDS.
 //   3. The upper triangle of D_pair is cut into 16x16 blocks dealt round-robin to
 //      the waves; each block is one v_mfma_f64_16x16x4f64 accumulator (FP64 matrix
 //      cores, four Z rows per instruction), updated by every chunk.
@@ -60,7 +60,6 @@ constexpr uint16_t NOBIN = 0xFFFF;
 constexpr int BS = 16;       // D_pair block edge: one v_mfma_f64_16x16x4f64 accumulator
 constexpr int BPW = 16;      // blocks per wave per pass over Z
 constexpr int KC_MAX = 64;   // Z rows per LDS chunk
-constexpr int SEG = 16;      // pixels walked by one thread while building a chunk
 
 template <class T>
 __device__ __forceinline__ const CAS T* cptr(const T* p) {
@@ -90,9 +89,9 @@ struct DArgs {
     uint32_t* queue;
 };
 
-// LDS bytes before the chunk buffer: bins, bin maps, chunk weights.
-__host__ __device__ constexpr size_t lds_fixed(int nb) {
-    return (((size_t)2 * MAX_PAIRS_LDS + 4 * (size_t)nb + 15) & ~(size_t)15) + 8 * KC_MAX;
+// LDS bytes before the chunk buffer: bins, bin maps, chunk weights, pixel weights.
+__host__ __device__ constexpr size_t lds_fixed(int nb, int max_len) {
+    return (((size_t)2 * MAX_PAIRS_LDS + 4 * (size_t)nb + 15) & ~(size_t)15) + 8 * KC_MAX + 32 * (size_t)max_len;
 }
 
 __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
@@ -103,7 +102,11 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
     int16_t* cmap = (int16_t*)(lds + 2 * MAX_PAIRS_LDS);               // [NB] bin -> compact, -1
     int16_t* act = cmap + NB;                                          // [NB] compact -> bin
     double* vk = (double*)(lds + (((size_t)2 * MAX_PAIRS_LDS + 4 * NB + 15) & ~(size_t)15));  // [KC_MAX]
-    double* zs = (double*)(lds + lds_fixed(NB));                      // [zcap]: Z chunk [kk][b]
+    double* sw1 = vk + KC_MAX;                                         // [4][max_len]: w, w dl of f1, f2
+    double* swd1 = sw1 + a.max_len;
+    double* sw2 = swd1 + a.max_len;
+    double* swd2 = sw2 + a.max_len;
+    double* zs = (double*)(lds + lds_fixed(NB, a.max_len));           // [zcap]: Z chunk [kk][b]
     const int tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
     const double bp64 = (double)a.np_rp / a.rpmax, bt64 = (double)a.np_rt / a.rtmax;
 
@@ -176,22 +179,45 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
         const int npad = NBK * BS;
         const int nblk = NBK * (NBK + 1) / 2;
         const int kc_max = min(KC_MAX, (a.zcap / npad) & ~1);
-        const int nseg1 = (n1 + SEG - 1) / SEG, nseg2 = (n2 + SEG - 1) / SEG;
+        for (int i = tid; i < n1; i += DTHREADS) {
+            const double w = a.we[p1 + i];
+            sw1[i] = w;
+            swd1[i] = w * a.dl[p1 + i];
+        }
+        for (int j = tid; j < n2; j += DTHREADS) {
+            const double w = a.we[p2 + j];
+            sw2[j] = w;
+            swd2[j] = w * a.dl[p2 + j];
+        }
+        // visits (r, c) for r < nr, c < nc in row-major order, strided over the workgroup
+        auto grid = [&](int nr, int nc, auto&& f) {
+            int r = tid / nc, c = tid - r * nc;
+            const int dr = DTHREADS / nc, dc = DTHREADS - dr * nc;
+            while (r < nr) {
+                f(r, c);
+                r += dr;
+                c += dc;
+                if (c >= nc) {
+                    c -= nc;
+                    ++r;
+                }
+            }
+        };
         const int nchunk_j = (2 * n2 + kc_max - 1) / kc_max, nchunk_i = (2 * n1 + kc_max - 1) / kc_max;
         const int nchunk = nchunk_j + nchunk_i + (kc_max >= 4 ? 1 : 2);
         for (int t0 = 0; t0 < nblk; t0 += DWAVES * BPW) {
-            // this wave's 16x16 blocks t0 + q*DWAVES + wave, as (row block, column block
-            // >= row block); the lane's A and B operand column within the block
+            // this wave's 16x16 blocks t0 + q*DWAVES + wave (q < nbw), as (row block,
+            // column block >= row block); the lane's A and B operand column in the block
             int boa[BPW], bob[BPW];
-            bool bv[BPW];
+            const int nbw = max(0, min(BPW, (nblk - t0 - wave + DWAVES - 1) / DWAVES));
 #pragma unroll
             for (int q = 0; q < BPW; ++q) {
                 int t = t0 + q * DWAVES + wave;
-                bv[q] = t < nblk;
                 int ba = 0;
-                while (bv[q] && t >= NBK - ba) { t -= NBK - ba; ++ba; }
-                boa[q] = ba * BS + (lane & 15);
-                bob[q] = (ba + t) * BS + (lane & 15);
+                while (q < nbw && t >= NBK - ba) { t -= NBK - ba; ++ba; }
+                // blocks past nbw read row 0 of the chunk and are never accumulated
+                boa[q] = q < nbw ? ba * BS + (lane & 15) : (lane & 15);
+                bob[q] = q < nbw ? (ba + t) * BS + (lane & 15) : (lane & 15);
             }
             double4v acc[BPW];
 #pragma unroll
@@ -211,25 +237,13 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
                         vk[tid] = (tid & 1) ? -wj * L1 : -wj * O1;
                     }
                     __syncthreads();
-                    for (int it = tid; it < nj * nseg1; it += DTHREADS) {
-                        const int jj = it / nseg1, i0 = (it - jj * nseg1) * SEG, i1 = min(n1, i0 + SEG);
-                        const uint16_t* row = binc + (j0 + jj) * n1;
-                        double* z0 = zs + (2 * jj) * npad;
-                        int cur = -1;
-                        double s0 = 0.0, s1 = 0.0;
-                        for (int i = i0; i < i1; ++i) {
-                            const int b = row[i];
-                            if (b == NOBIN) continue;
-                            if (b != cur) {
-                                if (cur >= 0) { atomicAdd(z0 + cur, s0); atomicAdd(z0 + npad + cur, s1); }
-                                cur = b; s0 = 0.0; s1 = 0.0;
-                            }
-                            const double wi = cptr(a.we)[p1 + i];
-                            s0 += wi * iO1;
-                            s1 += wi * cptr(a.dl)[p1 + i] * iL1;
+                    grid(nj, n1, [&](int jj, int i) {
+                        const int b = binc[(j0 + jj) * n1 + i];
+                        if (b != NOBIN) {
+                            atomicAdd(zs + (2 * jj) * npad + b, sw1[i] * iO1);
+                            atomicAdd(zs + (2 * jj + 1) * npad + b, swd1[i] * iL1);
                         }
-                        if (cur >= 0) { atomicAdd(z0 + cur, s0); atomicAdd(z0 + npad + cur, s1); }
-                    }
+                    });
                 } else if (ch < nchunk_j + nchunk_i) {
                     // rows (e12, e13) of forest-1 pixels i0 .. i0 + kc/2
                     const int i0 = (ch - nchunk_j) * (kc_max / 2);
@@ -241,25 +255,13 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
                         vk[tid] = (tid & 1) ? -wi * L2 : -wi * O2;
                     }
                     __syncthreads();
-                    for (int it = tid; it < ni * nseg2; it += DTHREADS) {
-                        const int ii = it / nseg2, j0 = (it - ii * nseg2) * SEG, j1 = min(n2, j0 + SEG);
-                        const uint16_t* col = binc + i0 + ii;
-                        double* z0 = zs + (2 * ii) * npad;
-                        int cur = -1;
-                        double s0 = 0.0, s1 = 0.0;
-                        for (int j = j0; j < j1; ++j) {
-                            const int b = col[j * n1];
-                            if (b == NOBIN) continue;
-                            if (b != cur) {
-                                if (cur >= 0) { atomicAdd(z0 + cur, s0); atomicAdd(z0 + npad + cur, s1); }
-                                cur = b; s0 = 0.0; s1 = 0.0;
-                            }
-                            const double wj = cptr(a.we)[p2 + j];
-                            s0 += wj * iO2;
-                            s1 += wj * cptr(a.dl)[p2 + j] * iL2;
+                    grid(n2, ni, [&](int j, int ii) {
+                        const int b = binc[j * n1 + i0 + ii];
+                        if (b != NOBIN) {
+                            atomicAdd(zs + (2 * ii) * npad + b, sw2[j] * iO2);
+                            atomicAdd(zs + (2 * ii + 1) * npad + b, swd2[j] * iL2);
                         }
-                        if (cur >= 0) { atomicAdd(z0 + cur, s0); atomicAdd(z0 + npad + cur, s1); }
-                    }
+                    });
                 } else {
                     // rank rows r0 .. r0 + kc of S00, S01, S10, S11 = sums over the pixel
                     // pairs of a bin of w_i w_j, w_i w_j dl_j, w_i dl_i w_j, w_i dl_i w_j dl_j
@@ -272,35 +274,19 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
                         vk[tid] = (q < 2 ? iO1 : iL1) * ((q & 1) ? iL2 : iO2);
                     }
                     __syncthreads();
-                    for (int it = tid; it < n2 * nseg1; it += DTHREADS) {
-                        const int j = it / nseg1, i0 = (it - j * nseg1) * SEG, i1 = min(n1, i0 + SEG);
-                        const uint16_t* row = binc + j * n1;
-                        const double wj = cptr(a.we)[p2 + j], wdj = wj * cptr(a.dl)[p2 + j];
-                        int cur = -1;
-                        double s0 = 0.0, s1 = 0.0;   // sum w_i, sum w_i dl_i over the run
-                        auto flush = [&]() {
+                    grid(n2, n1, [&](int j, int i) {
+                        const int b = binc[j * n1 + i];
+                        if (b != NOBIN) {
                             if (r0 == 0) {
-                                atomicAdd(zs + cur, s0 * wj);
-                                atomicAdd(zs + npad + cur, s0 * wdj);
+                                atomicAdd(zs + b, sw1[i] * sw2[j]);
+                                atomicAdd(zs + npad + b, sw1[i] * swd2[j]);
                             }
                             if (r0 + kc > 2) {
-                                atomicAdd(zs + (2 - r0) * npad + cur, s1 * wj);
-                                atomicAdd(zs + (3 - r0) * npad + cur, s1 * wdj);
+                                atomicAdd(zs + (2 - r0) * npad + b, swd1[i] * sw2[j]);
+                                atomicAdd(zs + (3 - r0) * npad + b, swd1[i] * swd2[j]);
                             }
-                        };
-                        for (int i = i0; i < i1; ++i) {
-                            const int b = row[i];
-                            if (b == NOBIN) continue;
-                            if (b != cur) {
-                                if (cur >= 0) flush();
-                                cur = b; s0 = 0.0; s1 = 0.0;
-                            }
-                            const double wi = cptr(a.we)[p1 + i];
-                            s0 += wi;
-                            s1 += wi * cptr(a.dl)[p1 + i];
                         }
-                        if (cur >= 0) flush();
-                    }
+                    });
                 }
                 __syncthreads();
                 if (ch == nchunk_j + nchunk_i && t0 == 0) {   // the chunk holding S00
@@ -311,17 +297,27 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
                         unsafeAtomicAdd(a.D + (int64_t)A * NB + A, zs[b]);
                     }
                 }
-                // rank-kc update of this wave's blocks: lane l supplies A[l%16][k] = Z[row][k] v[k]
-                // and B[k][l%16] = Z[col][k] for k = kk0 + l/16 (A = 0 past the chunk)
+                // rank-kc update of the wave's blocks q < nbw: lane l supplies A[l%16][k] =
+                // Z[row][k] v[k] and B[k][l%16] = Z[col][k] for k = kk0 + l/16 (A = 0 past the
+                // chunk); operands of four blocks are loaded before their MFMAs
                 for (int kk0 = 0; kk0 < kc; kk0 += 4) {
                     const int k = kk0 + (lane >> 4);
                     const bool kv = k < kc;
                     const double v = kv ? vk[k] : 0.0;
                     const double* zr = zs + (kv ? k : 0) * npad;
 #pragma unroll
-                    for (int q = 0; q < BPW; ++q) {
-                        if (!bv[q]) continue;
-                        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(zr[boa[q]] * v, zr[bob[q]], acc[q], 0, 0, 0);
+                    for (int g = 0; g < BPW; g += 4) {
+                        if (g >= nbw) break;
+                        double av[4], bw[4];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            av[q] = zr[boa[g + q]];
+                            bw[q] = zr[bob[g + q]];
+                        }
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            if (g + q < nbw)
+                                acc[g + q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q] * v, bw[q], acc[g + q], 0, 0, 0);
                     }
                 }
                 __syncthreads();
@@ -330,7 +326,7 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
             // accumulator element r of lane l is D_pair[l/16 + 4 r][l%16] of the block
 #pragma unroll
             for (int q = 0; q < BPW; ++q) {
-                if (!bv[q]) continue;
+                if (q >= nbw) continue;
                 const int col = bob[q];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
@@ -389,14 +385,14 @@ extern "C" int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, cons
         return lya_fail(LYA_ERR_HIP, "no HIP device");
     // the chunk buffer takes the rest of the workgroup's LDS; it must hold two rows of
     // every active bin (padded to the tile edge)
-    const size_t fixed = lds_fixed(NB) + 256;   // 256: the kernel's static LDS
+    const size_t fixed = lds_fixed(NB, max_len) + 256;   // 256: the kernel's static LDS
     const int64_t zcap = fixed < prop.sharedMemPerBlock ? (int64_t)((prop.sharedMemPerBlock - fixed) / 8) : 0;
     const int64_t npad_max = (int64_t)(NB + BS - 1) / BS * BS;
     if (zcap < 2 * npad_max)
         return lya_fail(LYA_ERR_UNSUPPORTED, "distortion: %d bins leave %lld doubles of LDS for Z chunks (< %lld)",
                         NB, (long long)zcap, (long long)(2 * npad_max));
     const int32_t zc = (int32_t)std::min<int64_t>(zcap, (int64_t)KC_MAX * npad_max);
-    const size_t lds = lds_fixed(NB) + (size_t)8 * zc;
+    const size_t lds = lds_fixed(NB, max_len) + (size_t)8 * zc;
     if (hipFuncSetAttribute((const void*)distortion_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
         hipSuccess)
         return lya_fail(LYA_ERR_HIP, "distortion: cannot reserve %zu bytes of LDS", lds);
