@@ -209,6 +209,7 @@ int lya_xi_adjoint(const lya_xi_problem* p, const lya_xi_config* cfg, const doub
  * (row <= column) is accumulated: call lya_distortion_mirror() after the last
  * lya_distortion_run() to copy it into the lower triangle.
  * scratch: device buffer of at least lya_distortion_scratch_bytes() bytes.
+ * groups: persistent workgroups (one per CU fills the GPU).
  * Replaces the kernels launched at distortion_procedures_pycuda.py:249-279.
  */
 int64_t lya_distortion_scratch_bytes(int32_t groups, int32_t max_len, int32_t nbins);

@@ -22,17 +22,18 @@
 // D_pair is symmetric, a rank-K (K = 2 n1 + 2 n2 + 4) product over the nact active
 // bins: nact^2 K / 2 multiply-adds instead of n1 n2 nact histogram updates.
 //
-// MI355X organisation (one workgroup of 8 waves per forest pair, persistent over
-// an atomic queue of pairs):
-//   1. FP64 bins of all pixel pairs (the xi formula) into LDS (u16, j-major);
-//      active bins flagged and compacted (wave scan) to b = 0..nact-1 in bin order.
-//   2. Z is built in chunks of at most 64 rows in LDS, zs[kk][b]: one thread per
-//      pixel pair of the chunk adds its weight into its bin (LDS atomics); the
-//      pixel weights w and w dl of both forests are staged in LDS.
+// MI355X organisation: one workgroup of 8 waves per CU, persistent over an atomic
+// queue of forest pairs.  Per pair:
+//   1. dc, w and w dl of both forests are staged in LDS; the FP64 bin of every
+//      pixel pair (the xi formula) flags the active bins, which a wave scan
+//      numbers 0..nact-1 in bin order.  No per-pair bin table is kept: bins are
+//      recomputed where needed, which leaves the LDS to the Z chunks.
+//   2. Z is built in chunks of at most 128 rows in LDS, zs[kk][b]: one thread per
+//      pixel pair of the chunk recomputes its bin and adds its weight (LDS atomics).
 //   3. The upper triangle of D_pair is cut into 16x16 blocks dealt round-robin to
 //      the waves; each block is one v_mfma_f64_16x16x4f64 accumulator (FP64 matrix
 //      cores, four Z rows per instruction), updated by every chunk.
-//   4. The tiles go into the upper triangle of D (FP64 global atomics, act is
+//   4. The blocks go into the upper triangle of D (FP64 global atomics, act is
 //      ascending so row <= column); lya_distortion_mirror() fills the lower
 //      triangle once accumulation is over.
 // Forests whose L = 0 (constant loglam) get zero slope terms (the reference
@@ -55,11 +56,10 @@ typedef double double4v __attribute__((ext_vector_type(4)));
 
 constexpr int DWAVES = 8;
 constexpr int DTHREADS = DWAVES * 64;
-constexpr int MAX_PAIRS_LDS = 44000;   // u16 bins of one forest pair held in LDS
-constexpr uint16_t NOBIN = 0xFFFF;
 constexpr int BS = 16;       // D_pair block edge: one v_mfma_f64_16x16x4f64 accumulator
 constexpr int BPW = 16;      // blocks per wave per pass over Z
-constexpr int KC_MAX = 64;   // Z rows per LDS chunk
+constexpr int KC_MAX = 128;  // Z rows per LDS chunk
+constexpr size_t LDS_STATIC = 256;         // the kernel's static LDS, rounded up
 
 template <class T>
 __device__ __forceinline__ const CAS T* cptr(const T* p) {
@@ -89,26 +89,44 @@ struct DArgs {
     uint32_t* queue;
 };
 
-// LDS bytes before the chunk buffer: bins, bin maps, chunk weights, pixel weights.
+// LDS bytes before the chunk buffer: bin maps, chunk weights, staged pixels.
 __host__ __device__ constexpr size_t lds_fixed(int nb, int max_len) {
-    return (((size_t)2 * MAX_PAIRS_LDS + 4 * (size_t)nb + 15) & ~(size_t)15) + 8 * KC_MAX + 32 * (size_t)max_len;
+    return ((4 * (size_t)nb + 15) & ~(size_t)15) + 8 * KC_MAX + 48 * (size_t)max_len;
 }
 
 __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int s_pair, s_nact;
     const int NB = a.np_rp * a.np_rt;
-    uint16_t* binc = (uint16_t*)lds;                                   // [MAX_PAIRS_LDS]
-    int16_t* cmap = (int16_t*)(lds + 2 * MAX_PAIRS_LDS);               // [NB] bin -> compact, -1
+    const int NP = a.np_rp, NT = a.np_rt;
+    int16_t* cmap = (int16_t*)lds;                                     // [NB] bin -> compact, -1
     int16_t* act = cmap + NB;                                          // [NB] compact -> bin
-    double* vk = (double*)(lds + (((size_t)2 * MAX_PAIRS_LDS + 4 * NB + 15) & ~(size_t)15));  // [KC_MAX]
-    double* sw1 = vk + KC_MAX;                                         // [4][max_len]: w, w dl of f1, f2
-    double* swd1 = sw1 + a.max_len;
-    double* sw2 = swd1 + a.max_len;
+    double* vk = (double*)(lds + ((4 * (size_t)NB + 15) & ~(size_t)15));  // [KC_MAX] chunk weights
+    double* r1s = vk + KC_MAX;                                         // [6][max_len] staged pixels
+    double* sw1 = r1s + a.max_len;                                     //   w of forest 1
+    double* swd1 = sw1 + a.max_len;                                    //   w dl of forest 1
+    double* r2s = swd1 + a.max_len;
+    double* sw2 = r2s + a.max_len;
     double* swd2 = sw2 + a.max_len;
     double* zs = (double*)(lds + lds_fixed(NB, a.max_len));           // [zcap]: Z chunk [kk][b]
     const int tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
-    const double bp64 = (double)a.np_rp / a.rpmax, bt64 = (double)a.np_rt / a.rtmax;
+    const double bp64 = (double)NP / a.rpmax, bt64 = (double)NT / a.rtmax;
+
+    // visits (r, c) for r < nr, c < nc in row-major order, strided over the workgroup
+    auto grid = [&](int nr, int nc, auto&& f) {
+        if (nr <= 0 || nc <= 0) return;
+        int r = tid / nc, c = tid - r * nc;
+        const int dr = DTHREADS / nc, dc = DTHREADS - dr * nc;
+        while (r < nr) {
+            f(r, c);
+            r += dr;
+            c += dc;
+            if (c >= nc) {
+                c -= nc;
+                ++r;
+            }
+        }
+    };
 
     for (;;) {
         if (tid == 0) s_pair = (int)atomicAdd(a.queue, 1u);
@@ -126,23 +144,33 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
         cos12 = cos12 + cptr(a.uz)[f1] * cptr(a.uz)[f2];
         cos12 = fmin(1.0, fmax(-1.0, cos12));
         const double c = sqrt(0.5 * (1.0 + cos12)), s = sqrt(0.5 * (1.0 - cos12));
-        const int npp = n1 * n2;
-
-        // ---- 1. bins, active flags -------------------------------------------------
-        for (int b = tid; b < NB; b += DTHREADS) cmap[b] = -1;
-        __syncthreads();
-        for (int k = tid; k < npp; k += DTHREADS) {
-            const int j = k / n1, i = k - j * n1;
-            const double r1 = a.dc[p1 + i], r2 = a.dc[p2 + j];
+        // global bin of pixel pair (i, j), the xi kernel's FP64 expression, or -1
+        auto gbin = [&](int i, int j) -> int {
+            const double r1 = r1s[i], r2 = r2s[j];
             const uint32_t bp = (uint32_t)__double2int_rz((fabs(r1 - r2) * c) * bp64);
             const uint32_t bt = (uint32_t)__double2int_rz(((r1 + r2) * s) * bt64);
-            uint16_t A = NOBIN;
-            if (bp < (uint32_t)a.np_rp && bt < (uint32_t)a.np_rt) {
-                A = (uint16_t)(bp * a.np_rt + bt);
-                cmap[A] = 1;
-            }
-            binc[k] = A;
+            return (bp < (uint32_t)NP && bt < (uint32_t)NT) ? (int)(bp * NT + bt) : -1;
+        };
+
+        // ---- 1. staging, active bins ------------------------------------------------
+        for (int i = tid; i < n1; i += DTHREADS) {
+            const double w = a.we[p1 + i];
+            r1s[i] = a.dc[p1 + i];
+            sw1[i] = w;
+            swd1[i] = w * a.dl[p1 + i];
         }
+        for (int j = tid; j < n2; j += DTHREADS) {
+            const double w = a.we[p2 + j];
+            r2s[j] = a.dc[p2 + j];
+            sw2[j] = w;
+            swd2[j] = w * a.dl[p2 + j];
+        }
+        for (int b = tid; b < NB; b += DTHREADS) cmap[b] = -1;
+        __syncthreads();
+        grid(n2, n1, [&](int j, int i) {
+            const int g = gbin(i, j);
+            if (g >= 0) cmap[g] = 1;
+        });
         __syncthreads();
         // compaction in bin order (wave 0)
         if (wave == 0) {
@@ -167,42 +195,18 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
         }
         __syncthreads();
         const int nact = uni(s_nact);
-        for (int k = tid; k < npp; k += DTHREADS) {
-            const uint16_t A = binc[k];
-            binc[k] = (A == NOBIN) ? NOBIN : (uint16_t)cmap[A];
-        }
-        __syncthreads();
         if (nact == 0) continue;
+        // compact bin of pixel pair (i, j), or -1
+        auto cbin = [&](int i, int j) -> int {
+            const int g = gbin(i, j);
+            return g >= 0 ? (int)cmap[g] : -1;
+        };
 
         // ---- 2. D_pair = Z diag(v) Z^T over the upper triangle ----------------------
         const int NBK = (nact + BS - 1) / BS;
         const int npad = NBK * BS;
         const int nblk = NBK * (NBK + 1) / 2;
         const int kc_max = min(KC_MAX, (a.zcap / npad) & ~1);
-        for (int i = tid; i < n1; i += DTHREADS) {
-            const double w = a.we[p1 + i];
-            sw1[i] = w;
-            swd1[i] = w * a.dl[p1 + i];
-        }
-        for (int j = tid; j < n2; j += DTHREADS) {
-            const double w = a.we[p2 + j];
-            sw2[j] = w;
-            swd2[j] = w * a.dl[p2 + j];
-        }
-        // visits (r, c) for r < nr, c < nc in row-major order, strided over the workgroup
-        auto grid = [&](int nr, int nc, auto&& f) {
-            int r = tid / nc, c = tid - r * nc;
-            const int dr = DTHREADS / nc, dc = DTHREADS - dr * nc;
-            while (r < nr) {
-                f(r, c);
-                r += dr;
-                c += dc;
-                if (c >= nc) {
-                    c -= nc;
-                    ++r;
-                }
-            }
-        };
         const int nchunk_j = (2 * n2 + kc_max - 1) / kc_max, nchunk_i = (2 * n1 + kc_max - 1) / kc_max;
         const int nchunk = nchunk_j + nchunk_i + (kc_max >= 4 ? 1 : 2);
         for (int t0 = 0; t0 < nblk; t0 += DWAVES * BPW) {
@@ -224,40 +228,40 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
             for (int q = 0; q < BPW; ++q) acc[q] = (double4v){0.0, 0.0, 0.0, 0.0};
 
             for (int ch = 0; ch < nchunk; ++ch) {
-                // build the chunk's rows of Z in zs[kk][a] and their weights in vk[kk]
+                // build the chunk's rows of Z in zs[kk][b] and their weights in vk[kk]
                 int kc;
                 if (ch < nchunk_j) {
-                    // rows (e21, e31) of forest-2 pixels j0 .. j0 + kc/2
+                    // rows (eta21, eta31) of forest-2 pixels j0 .. j0 + kc/2
                     const int j0 = ch * (kc_max / 2);
                     const int nj = min(kc_max / 2, n2 - j0);
                     kc = 2 * nj;
                     for (int e = tid; e < kc * npad; e += DTHREADS) zs[e] = 0.0;
                     if (tid < kc) {
-                        const double wj = cptr(a.we)[p2 + j0 + (tid >> 1)];
+                        const double wj = sw2[j0 + (tid >> 1)];
                         vk[tid] = (tid & 1) ? -wj * L1 : -wj * O1;
                     }
                     __syncthreads();
                     grid(nj, n1, [&](int jj, int i) {
-                        const int b = binc[(j0 + jj) * n1 + i];
-                        if (b != NOBIN) {
+                        const int b = cbin(i, j0 + jj);
+                        if (b >= 0) {
                             atomicAdd(zs + (2 * jj) * npad + b, sw1[i] * iO1);
                             atomicAdd(zs + (2 * jj + 1) * npad + b, swd1[i] * iL1);
                         }
                     });
                 } else if (ch < nchunk_j + nchunk_i) {
-                    // rows (e12, e13) of forest-1 pixels i0 .. i0 + kc/2
+                    // rows (eta12, eta13) of forest-1 pixels i0 .. i0 + kc/2
                     const int i0 = (ch - nchunk_j) * (kc_max / 2);
                     const int ni = min(kc_max / 2, n1 - i0);
                     kc = 2 * ni;
                     for (int e = tid; e < kc * npad; e += DTHREADS) zs[e] = 0.0;
                     if (tid < kc) {
-                        const double wi = cptr(a.we)[p1 + i0 + (tid >> 1)];
+                        const double wi = sw1[i0 + (tid >> 1)];
                         vk[tid] = (tid & 1) ? -wi * L2 : -wi * O2;
                     }
                     __syncthreads();
                     grid(n2, ni, [&](int j, int ii) {
-                        const int b = binc[j * n1 + i0 + ii];
-                        if (b != NOBIN) {
+                        const int b = cbin(i0 + ii, j);
+                        if (b >= 0) {
                             atomicAdd(zs + (2 * ii) * npad + b, sw2[j] * iO2);
                             atomicAdd(zs + (2 * ii + 1) * npad + b, swd2[j] * iL2);
                         }
@@ -275,8 +279,8 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
                     }
                     __syncthreads();
                     grid(n2, n1, [&](int j, int i) {
-                        const int b = binc[j * n1 + i];
-                        if (b != NOBIN) {
+                        const int b = cbin(i, j);
+                        if (b >= 0) {
                             if (r0 == 0) {
                                 atomicAdd(zs + b, sw1[i] * sw2[j]);
                                 atomicAdd(zs + npad + b, sw1[i] * swd2[j]);
@@ -371,9 +375,7 @@ extern "C" int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, cons
     if (np_rp < 1 || np_rt < 1 || !(rpmax > 0) || !(rtmax > 0)) return lya_fail(LYA_ERR_ARG, "distortion: bad binning");
     const int NB = np_rp * np_rt;
     if (NB > 16384) return lya_fail(LYA_ERR_UNSUPPORTED, "distortion: at most 16384 bins");
-    if (max_len < 1 || (int64_t)max_len * max_len > MAX_PAIRS_LDS)
-        return lya_fail(LYA_ERR_UNSUPPORTED, "distortion: max_len^2 = %lld exceeds %d pixel pairs per forest pair",
-                        (long long)max_len * max_len, MAX_PAIRS_LDS);
+    if (max_len < 1) return lya_fail(LYA_ERR_ARG, "distortion: max_len < 1");
     if (groups < 1) return lya_fail(LYA_ERR_ARG, "distortion: groups < 1");
     const int64_t need = lya_distortion_scratch_bytes(groups, max_len, NB);
     if (scratch_bytes < need)
@@ -383,14 +385,15 @@ extern "C" int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, cons
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
         return lya_fail(LYA_ERR_HIP, "no HIP device");
-    // the chunk buffer takes the rest of the workgroup's LDS; it must hold two rows of
-    // every active bin (padded to the tile edge)
-    const size_t fixed = lds_fixed(NB, max_len) + 256;   // 256: the kernel's static LDS
-    const int64_t zcap = fixed < prop.sharedMemPerBlock ? (int64_t)((prop.sharedMemPerBlock - fixed) / 8) : 0;
+    // the chunk buffer takes the rest of the workgroup's LDS; it must hold two rows
+    // of every active bin (padded to the block edge)
+    const size_t fixed = lds_fixed(NB, max_len) + LDS_STATIC;
     const int64_t npad_max = (int64_t)(NB + BS - 1) / BS * BS;
-    if (zcap < 2 * npad_max)
-        return lya_fail(LYA_ERR_UNSUPPORTED, "distortion: %d bins leave %lld doubles of LDS for Z chunks (< %lld)",
-                        NB, (long long)zcap, (long long)(2 * npad_max));
+    const size_t budget = prop.sharedMemPerBlock;
+    if (fixed + (size_t)16 * npad_max > budget)
+        return lya_fail(LYA_ERR_UNSUPPORTED, "distortion: %d bins and forests of %d pixels exceed the LDS", NB,
+                        max_len);
+    const int64_t zcap = (int64_t)((budget - fixed) / 8);
     const int32_t zc = (int32_t)std::min<int64_t>(zcap, (int64_t)KC_MAX * npad_max);
     const size_t lds = lds_fixed(NB, max_len) + (size_t)8 * zc;
     if (hipFuncSetAttribute((const void*)distortion_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=

@@ -70,3 +70,23 @@ def test_distortion_gpu_matches_oracle():
     np.testing.assert_allclose(w, w0, rtol=1e-12, atol=0)
     scale = np.abs(D0).max()
     assert np.max(np.abs(D - D0)) <= 1e-11 * scale, np.max(np.abs(D - D0)) / scale
+
+
+@pytest.mark.gpu
+def test_distortion_gpu_long_forests():
+    """Forests of 300-450 pixels (more pixel pairs than one LDS bin table would hold)
+    and pairs of very unequal length, against the oracle."""
+    fs = _short_set(40, 300, 450, 17)
+    am = forests.angmax_from(fs, 200.0)
+    off, idx = xi.build_pairs(fs, am)
+    f1 = np.repeat(np.arange(fs.nf), np.diff(off))
+    pairs = np.stack([f1, idx], 1)[::3][:60].astype(np.int32)
+    assert int(np.max(fs.n)) > 300 and len(pairs) > 10
+    D0, w0 = oracle.distortion(fs, pairs, form="eta")
+    eng = dist.DistortionEngine(fs, am, pairs=(off, idx))
+    eng.accumulate(pairs)
+    D, w = eng.result()
+    np.testing.assert_allclose(w, w0, rtol=1e-12, atol=0)
+    scale = np.abs(D0).max()
+    assert np.max(np.abs(D - D0)) <= 1e-11 * scale, np.max(np.abs(D - D0)) / scale
+    assert np.array_equal(D, D.T)
