@@ -199,6 +199,14 @@ int lya_xi_adjoint(const lya_xi_problem* p, const lya_xi_config* cfg, const doub
                    const double* beta, double* g_dw, double* g_w, void* stream);
 
 /*
+ * out[f] = sum of x[off[f] .. off[f+1]) for f < nseg (FP64, one wave per segment,
+ * fixed summation order).  The per-forest weighted sums of the projection
+ * (forest_class.py:54-66) in the gradient path, and the adjoint of broadcasting a
+ * per-forest value to its pixels.
+ */
+int lya_segment_sum(const double* x, const int64_t* off, int32_t nseg, double* out, void* stream);
+
+/*
  * Distortion matrix, FP64 accumulation.  Work items are forest pairs
  * {f1, f2} (already subsampled).  Per pair, with pixel bins A(i,j):
  *   D[A][B] += w_i w_j (eta terms of cuda_kernels.cpp:263-272) for every B

@@ -86,6 +86,7 @@ def lib():
     L.lya_distortion_run.argtypes = [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32,
                                      _i32, _i32, _f64, _f64, _vp, _vp, _vp, _i64, _i32, _vp]
     L.lya_distortion_mirror.argtypes = [_vp, _i32, _vp]
+    L.lya_segment_sum.argtypes = [_vp, _vp, _i32, _vp, _vp]
     L.lya_debug_halfangles.argtypes = [_vp, _vp, _vp, _i32, _vp]
     L.lya_copy_bytes.argtypes = [_vp, _vp, _i64, _vp]
     L.lya_build_tiles.argtypes = [_i64, _vp, _vp, _i64, _vp, _vp]
@@ -107,7 +108,7 @@ def check(rc):
 # Every symbol include/lya2pcf.h declares.
 EXPORTS = ("lya_last_error", "lya_version", "lya_build_pairs", "lya_build_work", "lya_xi_tau", "lya_xi_run",
            "lya_xi_graph_create", "lya_graph_launch", "lya_graph_destroy", "lya_xi_grid", "lya_xi_adjoint",
-           "lya_distortion_scratch_bytes", "lya_distortion_run", "lya_distortion_mirror", "lya_debug_halfangles", "lya_copy_bytes",
+           "lya_distortion_scratch_bytes", "lya_distortion_run", "lya_distortion_mirror", "lya_segment_sum", "lya_debug_halfangles", "lya_copy_bytes",
            "lya_build_tiles", "lya_pairs_device_scratch_bytes", "lya_build_pairs_device",
            "lya_items_device_scratch_bytes", "lya_build_items_device")
 

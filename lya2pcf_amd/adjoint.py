@@ -119,23 +119,74 @@ class WeightModel:
         return 1.0 / (eta * self.s2pip + s)
 
 
+def _segment_fns():
+    import torch
+
+    def seg_sum(x, off, nf):
+        if not (x.is_cuda and off.is_cuda):
+            raise ValueError("segment sum: device tensors expected")
+        x = x.contiguous()
+        out = torch.empty(nf, dtype=torch.float64, device=x.device)
+        check(lib().lya_segment_sum(ptr(x), ptr(off), nf, ptr(out), _lib.stream_ptr()))
+        return out
+
+    class SegmentSum(torch.autograd.Function):
+        """Per-forest sums of a pixel vector (HIP kernel); backward broadcasts."""
+
+        @staticmethod
+        def forward(ctx, x, proj):
+            ctx.proj = proj
+            return seg_sum(x.to(torch.float64), proj.off, proj.nf)
+
+        @staticmethod
+        def backward(ctx, g):
+            return g[ctx.proj.seg], None
+
+    class Broadcast(torch.autograd.Function):
+        """Per-forest values onto their pixels; backward sums per forest (HIP kernel)."""
+
+        @staticmethod
+        def forward(ctx, v, proj):
+            ctx.proj = proj
+            return v[proj.seg]
+
+        @staticmethod
+        def backward(ctx, g):
+            return seg_sum(g.to(torch.float64), ctx.proj.off, ctx.proj.nf), None
+
+    return SegmentSum, Broadcast
+
+
+_SEG = None
+
+
 class Projection:
     """dw = w (delta - <delta>_w - dl <dl delta>_w / <dl^2>_w), per forest."""
 
     def __init__(self, pix_off, loglam, device):
         import torch
+        global _SEG
+        if _SEG is None:
+            _SEG = _segment_fns()
         n = np.diff(pix_off)
         self.seg = torch.from_numpy(np.repeat(np.arange(len(n)), n)).to(device)
+        self.off = torch.from_numpy(np.asarray(pix_off, dtype=np.int64)).to(device)
         self.nf = len(n)
         self.loglam = torch.from_numpy(np.asarray(loglam, dtype=np.float64)).to(device)
 
     def _sum(self, x):
+        if self.off.is_cuda:
+            return _SEG[0].apply(x, self)
+        # host tensors (CPU tests of the projection algebra): torch's own segment sum
         import torch
-        return torch.zeros(self.nf, dtype=x.dtype, device=x.device).index_add_(0, self.seg, x)
+        return torch.zeros(self.nf, dtype=x.dtype).index_add_(0, self.seg, x)
+
+    def _bcast(self, v):
+        return _SEG[1].apply(v, self) if self.off.is_cuda else v[self.seg]
 
     def __call__(self, delta, w):
         om = self._sum(w)
-        avg = lambda x: (self._sum(w * x) / om)[self.seg]
+        avg = lambda x: self._bcast(self._sum(w * x) / om)
         dl = self.loglam - avg(self.loglam)
         slope = avg(dl * delta) / avg(dl * dl)
         return w * (delta - avg(delta) - dl * slope)

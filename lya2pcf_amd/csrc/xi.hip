@@ -1014,17 +1014,38 @@ __global__ void __launch_bounds__(WAVES * 64) xi_adjoint_kernel(const lya_xi_pro
                 if (jlo >= jhi || !valid) continue;
                 const double r1 = p.dc[p1 + i];
                 double gd = 0.0, gw = 0.0;
+                // in-range pairs gather alpha, beta of their bin; others read bin 0 times zero
+                auto gather = [&](uint32_t bp, uint32_t bt, double dwj, double wj) {
+                    const bool ok = bp < (uint32_t)NP && bt < (uint32_t)NT;
+                    const int A = ok ? (int)(bp * NT + bt) : 0;
+                    gd += ta[A] * (ok ? dwj : 0.0);
+                    gw += tb[A] * (ok ? wj : 0.0);
+                };
+                // forest-2 pixels in blocks of U wave-uniform scalar loads, as in the forward walk
+                constexpr int U = 8;
+                const CAS double* d2p = cptr(p.dw) + p2;
+                const CAS double* w2p = cptr(p.we) + p2;
                 if constexpr (MODE == LYA_MODE_FP64) {
-                    for (int j = jlo; j < jhi; ++j) {
-                        const double r2 = cptr(p.dc)[p2 + j];
-                        const uint32_t bp = (uint32_t)__double2int_rz((fabs(r1 - r2) * c) * bp64);
-                        const uint32_t bt = (uint32_t)__double2int_rz(((r1 + r2) * s) * bt64);
-                        if (bp < (uint32_t)NP && bt < (uint32_t)NT) {
-                            const int A = bp * NT + bt;
-                            gd += ta[A] * cptr(p.dw)[p2 + j];
-                            gw += tb[A] * cptr(p.we)[p2 + j];
+                    const CAS double* r2p = cptr(p.dc) + p2;
+                    auto block = [&](int j, auto UC) {
+                        constexpr int UU = decltype(UC)::value;
+                        double r2[UU], d2[UU], w2[UU];
+#pragma unroll
+                        for (int u = 0; u < UU; ++u) {
+                            r2[u] = r2p[j + u];
+                            d2[u] = d2p[j + u];
+                            w2[u] = w2p[j + u];
                         }
-                    }
+#pragma unroll
+                        for (int u = 0; u < UU; ++u) {
+                            const uint32_t bp = (uint32_t)__double2int_rz((fabs(r1 - r2[u]) * c) * bp64);
+                            const uint32_t bt = (uint32_t)__double2int_rz(((r1 + r2[u]) * s) * bt64);
+                            gather(bp, bt, d2[u], w2[u]);
+                        }
+                    };
+                    int j = jlo;
+                    for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
+                    for (; j < jhi; ++j) block(j, std::integral_constant<int, 1>{});
                 } else {
                     const double base2 = cptr(p.base)[f2];
                     const double cb = c * bp64, sb = s * bt64;
@@ -1032,28 +1053,64 @@ __global__ void __launch_bounds__(WAVES * 64) xi_adjoint_kernel(const lya_xi_pro
                     const float tau = p.tau, tau2 = 2.0f * tau;
                     const float P1 = (float)((r1 - base2) * cb);
                     const float T1 = (float)((r1 + base2) * sb + (double)tau);
-                    for (int j = jlo; j < jhi; ++j) {
-                        const float o2 = cptr(p.off32)[p2 + j];
-                        const float qp = __builtin_fabsf(__builtin_fmaf(o2, -c4, P1)) + tau;
-                        const float qt = __builtin_fmaf(o2, s4, T1);
-                        uint32_t bp = __float2uint_rz(qp), bt = __float2uint_rz(qt);
-                        if (fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt)) < tau2) {
-                            const double r2 = cptr(p.dc)[p2 + j];
-                            bp = (uint32_t)__double2int_rz((fabs(r1 - r2) * c) * bp64);
-                            bt = (uint32_t)__double2int_rz(((r1 + r2) * s) * bt64);
+                    const CAS float* o2p = cptr(p.off32) + p2;
+                    const CAS double* r2p = cptr(p.dc) + p2;
+                    auto block = [&](int j, auto UC) {
+                        constexpr int UU = decltype(UC)::value;
+                        float o2[UU], fr[UU];
+                        double d2[UU], w2[UU];
+                        uint32_t bp[UU], bt[UU];
+                        float frmin = 1.0f;
+#pragma unroll
+                        for (int u = 0; u < UU; ++u) {
+                            o2[u] = o2p[j + u];
+                            d2[u] = d2p[j + u];
+                            w2[u] = w2p[j + u];
                         }
-                        if (bp < (uint32_t)NP && bt < (uint32_t)NT) {
-                            const int A = bp * NT + bt;
-                            gd += ta[A] * cptr(p.dw)[p2 + j];
-                            gw += tb[A] * cptr(p.we)[p2 + j];
+#pragma unroll
+                        for (int u = 0; u < UU; ++u) {
+                            const float qp = __builtin_fabsf(__builtin_fmaf(o2[u], -c4, P1)) + tau;
+                            const float qt = __builtin_fmaf(o2[u], s4, T1);
+                            bp[u] = __float2uint_rz(qp);
+                            bt[u] = __float2uint_rz(qt);
+                            fr[u] = fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt));
+                            frmin = fminf(frmin, fr[u]);
                         }
-                    }
+                        if (frmin < tau2) {
+#pragma unroll
+                            for (int u = 0; u < UU; ++u) {
+                                if (fr[u] < tau2) {
+                                    const double r2 = r2p[j + u];
+                                    bp[u] = (uint32_t)__double2int_rz((fabs(r1 - r2) * c) * bp64);
+                                    bt[u] = (uint32_t)__double2int_rz(((r1 + r2) * s) * bt64);
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < UU; ++u) gather(bp[u], bt[u], d2[u], w2[u]);
+                    };
+                    int j = jlo;
+                    for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
+                    for (; j < jhi; ++j) block(j, std::integral_constant<int, 1>{});
                 }
                 unsafeAtomicAdd(g_dw + p1 + i, gd);
                 unsafeAtomicAdd(g_w + p1 + i, gw);
             }
         }
     }
+}
+
+// out[f] = sum of x[off[f] .. off[f+1]), one wave per segment, fixed summation order.
+__global__ void segment_sum_kernel(const double* __restrict__ x, const int64_t* __restrict__ off, int32_t nseg,
+                                   double* __restrict__ out) {
+    const int64_t seg = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (seg >= nseg) return;
+    const int64_t b = off[seg], e = off[seg + 1];
+    double acc = 0.0;
+    for (int64_t k = b + lane; k < e; k += 64) acc += x[k];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) out[seg] = acc;
 }
 
 __global__ void zero_outputs(uint64_t* cnt, double* w, double* dw, int64_t n, uint32_t* queue) {
@@ -1314,6 +1371,16 @@ extern "C" int lya_xi_tau(const lya_xi_problem* p, double max_abs_off, double ma
     };
     const double t = 2.0 * std::max(bound(p->np_rp, max_c_binner), bound(p->np_rt, max_s_binner));
     *tau = t;
+    return LYA_OK;
+}
+
+extern "C" int lya_segment_sum(const double* x, const int64_t* off, int32_t nseg, double* out, void* stream) {
+    if (nseg < 0 || (nseg > 0 && (!x || !off || !out))) return lya_fail(LYA_ERR_ARG, "segment sum: bad arguments");
+    if (nseg == 0) return LYA_OK;
+    const unsigned blocks = (unsigned)(((int64_t)nseg * 64 + 255) / 256);
+    hipLaunchKernelGGL(segment_sum_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, x, off, nseg, out);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return lya_fail(LYA_ERR_HIP, "segment sum launch: %s", hipGetErrorString(e));
     return LYA_OK;
 }
 

@@ -131,3 +131,21 @@ def test_gradient_wrt_delta_and_w_pixels():
                 fdv = (4 * d2 - d1) / 3
                 g = float(grad[k])
                 assert abs(g - fdv) <= 1e-5 * abs(fdv) + 1e-9 * float(chi2(delta0, w0)), (which, k, g, fdv)
+
+
+@pytest.mark.gpu
+def test_projection_device_matches_host():
+    """The device projection (HIP segment sums) against torch's own segment sums on
+    the host: values and gradients w.r.t. delta and w."""
+    import torch
+    s = synth.generate(synth.scaled(synth.CONFIGS["c0"], 60, seed=8))
+    out = {}
+    for dev in ("cpu", "cuda"):
+        pr = adjoint.Projection(s.pix_off, s.loglam, dev)
+        delta = torch.from_numpy(s.delta).to(dev).requires_grad_(True)
+        w = torch.from_numpy(s.we).to(dev).requires_grad_(True)
+        dw = pr(delta, w)
+        (dw * torch.cos(torch.arange(len(dw), device=dev, dtype=torch.float64))).sum().backward()
+        out[dev] = [t.detach().cpu().numpy() for t in (dw, delta.grad, w.grad)]
+    for a, b in zip(out["cpu"], out["cuda"]):
+        np.testing.assert_allclose(b, a, rtol=1e-11, atol=1e-13 * np.abs(a).max())
