@@ -101,21 +101,62 @@ def pair_sums(dw, w, eng):
     return _PairSums.apply(dw, w, eng)
 
 
+def _interp_fn():
+    import torch
+
+    class NodeInterp(torch.autograd.Function):
+        """s_i = (1 - f_i) v[k_i] + f_i v[k_i + 1]; backward sums per node over the
+        pixels pre-sorted by k (no scatter of millions of values into a few nodes)."""
+
+        @staticmethod
+        def forward(ctx, v, wm):
+            ctx.wm = wm
+            return (1.0 - wm.f) * v[wm.k0] + wm.f * v[wm.k0 + 1]
+
+        @staticmethod
+        def backward(ctx, g):
+            wm = ctx.wm
+            a = (g * (1.0 - wm.f))[wm.perm]
+            b = (g * wm.f)[wm.perm]
+            parts = []
+            for k in range(wm.n_nodes):
+                t = g.new_zeros(())
+                if k < wm.n_nodes - 1 and wm.offs[k + 1] > wm.offs[k]:
+                    t = t + a[wm.offs[k]:wm.offs[k + 1]].sum()
+                if k > 0 and wm.offs[k] > wm.offs[k - 1]:
+                    t = t + b[wm.offs[k - 1]:wm.offs[k]].sum()
+                parts.append(t)
+            return torch.stack(parts), None
+
+    return NodeInterp
+
+
+_INTERP = None
+
+
 class WeightModel:
     """w = 1 / (eta * sigma2_pip + sigma2_lss(z)), sigma2_lss linear between nodes."""
 
     def __init__(self, z, sigma2_pip, nodes, device):
         import torch
+        global _INTERP
+        if _INTERP is None:
+            _INTERP = _interp_fn()
         nodes = np.asarray(nodes, dtype=np.float64)
         zc = np.clip(z, nodes[0], nodes[-1])
         k0 = np.clip(np.searchsorted(nodes, zc, side="right") - 1, 0, len(nodes) - 2)
         f = (zc - nodes[k0]) / (nodes[k0 + 1] - nodes[k0])
+        self.n_nodes = len(nodes)
+        perm = np.argsort(k0, kind="stable")
+        # pixels of interval k (between nodes k and k+1) are perm[offs[k]:offs[k+1]]
+        self.offs = [int(x) for x in np.searchsorted(k0[perm], np.arange(len(nodes)), side="left")]
+        self.perm = torch.from_numpy(perm.astype(np.int64)).to(device)
         self.k0 = torch.from_numpy(k0.astype(np.int64)).to(device)
         self.f = torch.from_numpy(f).to(device)
         self.s2pip = torch.from_numpy(np.asarray(sigma2_pip, dtype=np.float64)).to(device)
 
     def __call__(self, eta, s2lss):
-        s = (1.0 - self.f) * s2lss[self.k0] + self.f * s2lss[self.k0 + 1]
+        s = _INTERP.apply(s2lss, self)
         return 1.0 / (eta * self.s2pip + s)
 
 

@@ -149,3 +149,21 @@ def test_projection_device_matches_host():
         out[dev] = [t.detach().cpu().numpy() for t in (dw, delta.grad, w.grad)]
     for a, b in zip(out["cpu"], out["cuda"]):
         np.testing.assert_allclose(b, a, rtol=1e-11, atol=1e-13 * np.abs(a).max())
+
+
+def test_weight_model_gradient_cpu():
+    """The node interpolation's hand-written backward (per-node sums over pixels
+    sorted by interval) against torch autograd of the same expression."""
+    import torch
+    s = synth.generate(synth.scaled(synth.CONFIGS["c0"], 40, seed=6))
+    wm = adjoint.WeightModel(s.z, s.sigma2_pip, s.config.z_nodes, "cpu")
+    g = torch.cos(torch.arange(len(s.z), dtype=torch.float64))
+    v = torch.linspace(0.03, 0.08, 10, dtype=torch.float64).requires_grad_(True)
+    eta = torch.tensor(1.1, dtype=torch.float64, requires_grad=True)
+    (wm(eta, v) * g).sum().backward()
+    v2 = v.detach().clone().requires_grad_(True)
+    e2 = eta.detach().clone().requires_grad_(True)
+    s2 = (1.0 - wm.f) * v2[wm.k0] + wm.f * v2[wm.k0 + 1]
+    (g / (e2 * wm.s2pip + s2)).sum().backward()
+    np.testing.assert_allclose(v.grad.numpy(), v2.grad.numpy(), rtol=1e-13, atol=1e-14 * float(v2.grad.abs().max()))
+    assert abs(float(eta.grad - e2.grad)) <= 1e-13 * abs(float(e2.grad))
