@@ -30,6 +30,9 @@
 //      recomputed where needed, which leaves the LDS to the Z chunks.
 //   2. Z is built in chunks of at most 128 rows in LDS, zs[kk][b]: one thread per
 //      pixel pair of the chunk recomputes its bin and adds its weight (LDS atomics).
+//      The four rank rows follow from the eta21, eta31 rows of the j chunks
+//      (S00[A] = O1 sum_j w_j eta21[A][j], ...) without another pass over the pairs;
+//      they are kept in a small per-workgroup global scratch.
 //   3. The upper triangle of D_pair is cut into 16x16 blocks dealt round-robin to
 //      the waves; each block is one v_mfma_f64_16x16x4f64 accumulator (FP64 matrix
 //      cores, four Z rows per instruction), updated by every chunk.
@@ -86,6 +89,8 @@ struct DArgs {
     double* D;
     double* wA;
     int32_t zcap;        // doubles of the LDS chunk buffer
+    int32_t npad_max;    // bins padded to the block edge
+    double* tscr;        // [groups][4][npad_max]: rank rows of each workgroup's current pair
     uint32_t* queue;
 };
 
@@ -206,9 +211,13 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
         const int NBK = (nact + BS - 1) / BS;
         const int npad = NBK * BS;
         const int nblk = NBK * (NBK + 1) / 2;
+        // chunk rows in zs[0 .. kc_max npad); the four rank rows T00, T01, T10, T11 (see the
+        // rank chunks below) in the workgroup's global scratch, T[q npad + b]
+        double* T = a.tscr + (int64_t)blockIdx.x * 4 * a.npad_max;
         const int kc_max = min(KC_MAX, (a.zcap / npad) & ~1);
         const int nchunk_j = (2 * n2 + kc_max - 1) / kc_max, nchunk_i = (2 * n1 + kc_max - 1) / kc_max;
-        const int nchunk = nchunk_j + nchunk_i + (kc_max >= 4 ? 1 : 2);
+        const int nrank = kc_max >= 4 ? 1 : 2;   // rank chunks of 4 rows, or of 2
+        const int nchunk = nchunk_j + nchunk_i + nrank;
         for (int t0 = 0; t0 < nblk; t0 += DWAVES * BPW) {
             // this wave's 16x16 blocks t0 + q*DWAVES + wave (q < nbw), as (row block,
             // column block >= row block); the lane's A and B operand column in the block
@@ -228,7 +237,7 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
             for (int q = 0; q < BPW; ++q) acc[q] = (double4v){0.0, 0.0, 0.0, 0.0};
 
             for (int ch = 0; ch < nchunk; ++ch) {
-                // build the chunk's rows of Z in zs[kk][b] and their weights in vk[kk]
+                // build the chunk's rows of Z in zb[kk][b] and their weights in vk[kk]
                 int kc;
                 if (ch < nchunk_j) {
                     // rows (eta21, eta31) of forest-2 pixels j0 .. j0 + kc/2
@@ -236,6 +245,8 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
                     const int nj = min(kc_max / 2, n2 - j0);
                     kc = 2 * nj;
                     for (int e = tid; e < kc * npad; e += DTHREADS) zs[e] = 0.0;
+                    if (ch == 0 && t0 == 0)
+                        for (int e = tid; e < 4 * npad; e += DTHREADS) T[e] = 0.0;
                     if (tid < kc) {
                         const double wj = sw2[j0 + (tid >> 1)];
                         vk[tid] = (tid & 1) ? -wj * L1 : -wj * O1;
@@ -267,38 +278,46 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
                         }
                     });
                 } else {
-                    // rank rows r0 .. r0 + kc of S00, S01, S10, S11 = sums over the pixel
-                    // pairs of a bin of w_i w_j, w_i w_j dl_j, w_i dl_i w_j, w_i dl_i w_j dl_j
-                    // (one chunk, or two of two rows when the LDS holds only two rows)
-                    const int r0 = (ch - nchunk_j - nchunk_i) * 2 * (kc_max >= 4 ? 2 : 1);
-                    kc = kc_max >= 4 ? 4 : 2;
-                    for (int e = tid; e < kc * npad; e += DTHREADS) zs[e] = 0.0;
+                    // rank rows.  With S00, S01, S10, S11 the bin sums of w_i w_j, w_i w_j dl_j,
+                    // w_i dl_i w_j, w_i dl_i w_j dl_j:  S00 = O1 T00, S01 = O1 T01, S10 = L1 T10,
+                    // S11 = L1 T11, where T00[b] = sum_j w_j eta21[b][j], T01 = sum_j w_j dl_j eta21,
+                    // T10 = sum_j w_j eta31, T11 = sum_j w_j dl_j eta31 were accumulated from the
+                    // j chunks.  S_q[A] S_q[B] v_q becomes T_q[A] T_q[B] (v_q times the factors).
+                    kc = nrank == 1 ? 4 : 2;
+                    const int r0 = (ch - nchunk_j - nchunk_i) * 2;
+                    for (int e = tid; e < kc * npad; e += DTHREADS) zs[e] = T[r0 * npad + e];
                     if (tid < kc) {
                         const int q = r0 + tid;
-                        vk[tid] = (q < 2 ? iO1 : iL1) * ((q & 1) ? iL2 : iO2);
+                        vk[tid] = (q < 2 ? O1 : L1) * ((q & 1) ? iL2 : iO2);
                     }
-                    __syncthreads();
-                    grid(n2, n1, [&](int j, int i) {
-                        const int b = cbin(i, j);
-                        if (b >= 0) {
-                            if (r0 == 0) {
-                                atomicAdd(zs + b, sw1[i] * sw2[j]);
-                                atomicAdd(zs + npad + b, sw1[i] * swd2[j]);
-                            }
-                            if (r0 + kc > 2) {
-                                atomicAdd(zs + (2 - r0) * npad + b, swd1[i] * sw2[j]);
-                                atomicAdd(zs + (3 - r0) * npad + b, swd1[i] * swd2[j]);
-                            }
-                        }
-                    });
                 }
                 __syncthreads();
-                if (ch == nchunk_j + nchunk_i && t0 == 0) {   // the chunk holding S00
+                if (ch < nchunk_j && t0 == 0) {
+                    // rank rows from this chunk's eta21, eta31 rows (each thread owns its bins)
+                    const int j0 = ch * (kc_max / 2);
+                    for (int b = tid; b < nact; b += DTHREADS) {
+                        double t00 = T[b], t01 = T[npad + b], t10 = T[2 * npad + b], t11 = T[3 * npad + b];
+                        for (int jj = 0; 2 * jj < kc; ++jj) {
+                            const double z21 = zs[(2 * jj) * npad + b], z31 = zs[(2 * jj + 1) * npad + b];
+                            const double wj = sw2[j0 + jj], wdj = swd2[j0 + jj];
+                            t00 += wj * z21;
+                            t01 += wdj * z21;
+                            t10 += wj * z31;
+                            t11 += wdj * z31;
+                        }
+                        T[b] = t00;
+                        T[npad + b] = t01;
+                        T[2 * npad + b] = t10;
+                        T[3 * npad + b] = t11;
+                    }
+                }
+                if (ch == nchunk_j + nchunk_i && t0 == 0) {
                     // identity term and weights: D[A][A] += S00[A], wA[A] += S00[A]
                     for (int b = tid; b < nact; b += DTHREADS) {
                         const int A = act[b];
-                        unsafeAtomicAdd(a.wA + A, zs[b]);
-                        unsafeAtomicAdd(a.D + (int64_t)A * NB + A, zs[b]);
+                        const double s00 = O1 * zs[b];
+                        unsafeAtomicAdd(a.wA + A, s00);
+                        unsafeAtomicAdd(a.D + (int64_t)A * NB + A, s00);
                     }
                 }
                 // rank-kc update of the wave's blocks q < nbw: lane l supplies A[l%16][k] =
@@ -359,7 +378,8 @@ __global__ void mirror_kernel(double* D, int nb) {
 
 extern "C" int64_t lya_distortion_scratch_bytes(int32_t groups, int32_t max_len, int32_t nbins) {
     if (groups < 1 || max_len < 1 || nbins < 1) return -1;
-    return 64;
+    const int64_t npad = ((int64_t)nbins + BS - 1) / BS * BS;
+    return 256 + (int64_t)groups * 4 * npad * 8;   // pair queue, then the rank rows per workgroup
 }
 
 extern "C" int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, const double* dc, const double* we,
@@ -385,8 +405,8 @@ extern "C" int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, cons
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
         return lya_fail(LYA_ERR_HIP, "no HIP device");
-    // the chunk buffer takes the rest of the workgroup's LDS; it must hold two rows
-    // of every active bin (padded to the block edge)
+    // the chunk buffer takes the rest of the workgroup's LDS; it must hold two rows of
+    // every active bin (padded to the block edge)
     const size_t fixed = lds_fixed(NB, max_len) + LDS_STATIC;
     const int64_t npad_max = (int64_t)(NB + BS - 1) / BS * BS;
     const size_t budget = prop.sharedMemPerBlock;
@@ -394,7 +414,7 @@ extern "C" int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, cons
         return lya_fail(LYA_ERR_UNSUPPORTED, "distortion: %d bins and forests of %d pixels exceed the LDS", NB,
                         max_len);
     const int64_t zcap = (int64_t)((budget - fixed) / 8);
-    const int32_t zc = (int32_t)std::min<int64_t>(zcap, (int64_t)KC_MAX * npad_max);
+    const int32_t zc = (int32_t)std::min<int64_t>(zcap, (int64_t)(KC_MAX + 4) * npad_max);
     const size_t lds = lds_fixed(NB, max_len) + (size_t)8 * zc;
     if (hipFuncSetAttribute((const void*)distortion_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
         hipSuccess)
@@ -404,7 +424,9 @@ extern "C" int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, cons
     a.omega = omega; a.odl2 = odl2; a.pairs = pairs; a.n_pairs = n_pairs; a.max_len = max_len;
     a.np_rp = np_rp; a.np_rt = np_rt; a.rpmax = rpmax; a.rtmax = rtmax; a.D = D; a.wA = wA;
     a.zcap = zc;
+    a.npad_max = (int32_t)npad_max;
     a.queue = (uint32_t*)scratch;
+    a.tscr = (double*)((unsigned char*)scratch + 256);
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(zero_queue, dim3(1), dim3(64), 0, st, a.queue);
     if (n_pairs > 0) hipLaunchKernelGGL(distortion_kernel, dim3(groups), dim3(DTHREADS), lds, st, a);

@@ -90,3 +90,37 @@ def test_distortion_gpu_long_forests():
     scale = np.abs(D0).max()
     assert np.max(np.abs(D - D0)) <= 1e-11 * scale, np.max(np.abs(D - D0)) / scale
     assert np.array_equal(D, D.T)
+
+
+def _active_bins(fs, f1, f2, np_rp, np_rt, rpmax, rtmax):
+    r1 = fs.dc[fs.pix_off[f1]:fs.pix_off[f1 + 1]]
+    r2 = fs.dc[fs.pix_off[f2]:fs.pix_off[f2 + 1]]
+    cos = fs.ux[f1] * fs.ux[f2] + fs.uy[f1] * fs.uy[f2]
+    cos = min(1.0, max(-1.0, cos + fs.uz[f1] * fs.uz[f2]))
+    c, s = np.sqrt(0.5 * (1 + cos)), np.sqrt(0.5 * (1 - cos))
+    bp = np.floor(np.abs(r1[:, None] - r2[None, :]) * c * (np_rp / rpmax))
+    bt = np.floor((r1[:, None] + r2[None, :]) * s * (np_rt / rtmax))
+    ok = (bp < np_rp) & (bt < np_rt)
+    return len(np.unique((bp * np_rt + bt)[ok]))
+
+
+@pytest.mark.gpu
+def test_distortion_gpu_many_active_bins():
+    """100 x 40 bins: forest pairs with more than 240 active bins take several passes
+    over their Z rows (the 16x16 blocks of one pass do not cover D_pair)."""
+    np_rp, np_rt, rmax = 100, 40, 200.0
+    fs = _short_set(60, 150, 208, 23)
+    am = forests.angmax_from(fs, 200.0)
+    off, idx = xi.build_pairs(fs, am)
+    f1 = np.repeat(np.arange(fs.nf), np.diff(off))
+    allp = np.stack([f1, idx], 1).astype(np.int32)
+    nact = np.array([_active_bins(fs, a, b, np_rp, np_rt, rmax, rmax) for a, b in allp])
+    pairs = allp[np.argsort(-nact)[:24]]
+    assert nact.max() > 240
+    D0, w0 = oracle.distortion(fs, pairs, np_rp, np_rt, rmax, rmax, form="eta")
+    eng = dist.DistortionEngine(fs, am, np_rp, np_rt, rmax, rmax, pairs=(off, idx))
+    eng.accumulate(pairs)
+    D, w = eng.result()
+    np.testing.assert_allclose(w, w0, rtol=1e-12, atol=0)
+    scale = np.abs(D0).max()
+    assert np.max(np.abs(D - D0)) <= 1e-11 * scale, np.max(np.abs(D - D0)) / scale
