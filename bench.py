@@ -277,7 +277,11 @@ def main():
                    "outputs": "sum w1w2, sum dw1dw2" + (" and pair counts" if args.counts else ""),
                    "parallelism": "single GPU, persistent work queue"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": autotune.measured_traffic(main_mode),
+                     "frac": achieved / peak,
+                     # HBM bytes per launch measured with rocprofv3 counters on the 100k-forest
+                     # workload only; other workloads report none
+                     "traffic": autotune.measured_traffic(main_mode) if args.config == "c1" and
+                     not args.forests else None,
                      "note": "10 FLOP per pixel pair (algorithmic) / median per-launch hipEvent time; "
                              "no MFMA or HBM bound applies"},
         "kernels": {k: {"value": P / v["wall"], "ms_per_step": v["wall"] * 1e3,

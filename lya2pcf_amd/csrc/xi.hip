@@ -224,8 +224,8 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                     an = 0u;
                 }
                 pk = key;
-                aw += w1k * w2[u];
-                ad += d1k * d2[u];
+                aw = fma(w1k, w2[u], aw);   // run sums: fused, like any FP64 partial sum here
+                ad = fma(d1k, d2[u], ad);
                 ++an;
             }
             if (pk != 0xFFFFFFFFu) flush();
@@ -891,7 +891,6 @@ __global__ void __launch_bounds__(WAVES * 64) xi_walk_kernel(const lya_xi_proble
                         const uint32_t kt = min((uint32_t)__double2int_rz(qt), (uint32_t)NT);
                         uint32_t key = __umul24(kp, (uint32_t)ntpad) + kt;
                         if constexpr (decltype(CHK)::value) key = ((t >= a) & (t < b)) ? key : TRASH;
-                        const double pw = w1 * w2, pd = d1 * d2;
                         if (key != prev) {  // the run of `prev` ended
                             if constexpr (CNT) lds_add<uint32_t>(lds, 4u * prev, an);
                             lds_add<double>(lds, L::W + 8u * prev, aw);
@@ -900,8 +899,8 @@ __global__ void __launch_bounds__(WAVES * 64) xi_walk_kernel(const lya_xi_proble
                             ad = 0.0;
                             an = 0u;
                         }
-                        aw += pw;
-                        ad += pd;
+                        aw = fma(w1, w2, aw);   // run sums, fused multiply-add
+                        ad = fma(d1, d2, ad);
                         if constexpr (CNT) ++an;
                         prev = key;
                     };
