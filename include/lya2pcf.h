@@ -212,6 +212,8 @@ int lya_segment_sum(const double* x, const int64_t* off, int32_t nseg, double* o
  *   D[A][B] += w_i w_j (eta terms of cuda_kernels.cpp:263-272) for every B
  *   active in the pair, D[A][A] += w_i w_j, wA[A] += w_i w_j.
  * dl = delta_lambda, omega = sum w, odl2 = sum w dl^2 per forest (FP64).
+ * max_len must be at least the length of every forest in the pairs (it sizes the
+ * LDS staging); pairs with a longer forest are skipped.
  * D: [nb][nb] FP64 with nb = np_rp*np_rt, wA: [nb]; both accumulated.
  * Each pair's contribution is symmetric, and only the upper triangle of D
  * (row <= column) is accumulated: call lya_distortion_mirror() after the last

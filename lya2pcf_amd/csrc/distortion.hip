@@ -141,6 +141,13 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
         const int f1 = cptr(a.pairs)[2 * pr], f2 = cptr(a.pairs)[2 * pr + 1];
         const int64_t p1 = cptr(a.pix_off)[f1], p2 = cptr(a.pix_off)[f2];
         const int n1 = (int)(cptr(a.pix_off)[f1 + 1] - p1), n2 = (int)(cptr(a.pix_off)[f2 + 1] - p2);
+        if (n1 > a.max_len || n2 > a.max_len) {
+            // longer than the staging arrays: skipped, and counted in queue[1]; the barrier
+            // keeps thread 0 from drawing the next pair before every wave has read this one
+            if (tid == 0) atomicAdd(a.queue + 1, 1u);
+            __syncthreads();
+            continue;
+        }
         const double O1 = cptr(a.omega)[f1], O2 = cptr(a.omega)[f2];
         const double L1 = cptr(a.odl2)[f1], L2 = cptr(a.odl2)[f2];
         const double iO1 = 1.0 / O1, iO2 = 1.0 / O2;
