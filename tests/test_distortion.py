@@ -124,3 +124,27 @@ def test_distortion_gpu_many_active_bins():
     np.testing.assert_allclose(w, w0, rtol=1e-12, atol=0)
     scale = np.abs(D0).max()
     assert np.max(np.abs(D - D0)) <= 1e-11 * scale, np.max(np.abs(D - D0)) / scale
+
+
+@pytest.mark.gpu
+def test_distortion_gpu_short_and_flat_forests():
+    """Forests of 2-6 pixels, every fifth with dl = 0 (L = sum w dl^2 = 0, the zero-slope
+    convention of both the oracle and the kernel), and pairs without any in-range pixel pair."""
+    cfg = synth.scaled(synth.CONFIGS["c0"], 200, seed=31, length_range=(1, 6))
+    fs = forests.ForestSet.from_survey(synth.generate(cfg))
+    for f in range(0, fs.nf, 5):
+        fs.dl[fs.pix_off[f]:fs.pix_off[f + 1]] = 0.0
+        fs.odl2[f] = 0.0
+    am = forests.angmax_from(fs, 200.0)
+    off, idx = xi.build_pairs(fs, am)
+    f1 = np.repeat(np.arange(fs.nf), np.diff(off))
+    pairs = np.stack([f1, idx], 1)[::11].astype(np.int32)
+    nact = np.array([_active_bins(fs, a, b, 50, 50, 200.0, 200.0) for a, b in pairs])
+    assert (nact == 0).any() and (fs.odl2 == 0).sum() >= 30
+    D0, w0 = oracle.distortion(fs, pairs, form="eta")
+    eng = dist.DistortionEngine(fs, am, pairs=(off, idx))
+    eng.accumulate(pairs)
+    D, w = eng.result()
+    np.testing.assert_allclose(w, w0, rtol=1e-12, atol=0)
+    scale = np.abs(D0).max()
+    assert np.max(np.abs(D - D0)) <= 1e-11 * scale, np.max(np.abs(D - D0)) / scale
