@@ -70,3 +70,19 @@ def test_tau_bound_small():
     t = C.c_double()
     _lib.check(_lib.lib().lya_xi_tau(C.byref(p), 210.0, 0.25, 0.25, C.byref(t)))
     assert 1e-6 < t.value < 1e-4
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """Without the HIP library the product path raises; there is no fallback."""
+    import subprocess
+    import sys
+    code = ("from lya2pcf_amd import _lib, xi\n"
+            "try:\n"
+            "    _lib.lib()\n"
+            "except _lib.LyaError as e:\n"
+            "    print('raised', e)\n")
+    env = dict(os.environ, LYA2PCF_LIB=str(tmp_path / "absent" / "liblya2pcf.so"))
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.startswith("raised") and "not built" in out.stdout
