@@ -201,8 +201,13 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
         // Sums of the products of lane pixel k with a block's forest-2 pixels, one set
         // of LDS atomics per run of equal bins (consecutive forest-2 pixels of a lane
         // often share a bin).
-        auto run_sums_impl = [&](auto UC, const uint32_t* bpk, const uint32_t* btk, double w1k,
-                                 const double* w2, double d1k, const double* d2) {
+        // LDS index of bin (bp, bt), or NONE when the pair is out of range
+        constexpr uint32_t NONE = 0xFFFFFFFFu;
+        auto bin_key = [&](uint32_t bp, uint32_t bt) -> uint32_t {
+            return (bp < (uint32_t)NP && bt < (uint32_t)NT) ? __umul24(bp, ntpad4 >> 2) + bt : NONE;
+        };
+        auto run_sums_impl = [&](auto UC, const uint32_t* keys, double w1k, const double* w2, double d1k,
+                                 const double* d2) {
             constexpr int UU = decltype(UC)::value;
             uint32_t pk = 0xFFFFFFFFu, an = 0u;
             double aw = 0.0, ad = 0.0;
@@ -215,8 +220,7 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
             };
 #pragma unroll
             for (int u = 0; u < UU; ++u) {
-                const uint32_t key = (bpk[u] < (uint32_t)NP && btk[u] < (uint32_t)NT)
-                                         ? __umul24(bpk[u], ntpad4 >> 2) + btk[u] : 0xFFFFFFFFu;
+                const uint32_t key = keys[u];
                 if (key != pk) {
                     if (pk != 0xFFFFFFFFu) flush();
                     aw = 0.0;
@@ -245,6 +249,15 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                 }
             }
         };
+        auto accumulate_key = [&](uint32_t key, double pw, double pd) {
+            if (key != NONE) {
+                const uint32_t a4 = (key << 2) + hbase;
+                if constexpr (CNT) lds_add<uint32_t>(lds, a4, 1u);
+                const uint32_t a8 = (a4 << 1) + woff;
+                lds_add<double>(lds, a8, pw);
+                lds_add<double>(lds, a8 + (L::DW - L::W), pd);
+            }
+        };
         if constexpr (MODE == LYA_MODE_FP64) {
             const double cq = POW2 ? c * bp64 : c, sq = POW2 ? s * bt64 : s;
             auto block = [&](int j, auto UC) {
@@ -260,7 +273,7 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                 if constexpr (MERGE) {
 #pragma unroll
                 for (int k = 0; k < PPL; ++k) {
-                    uint32_t bpk[UU], btk[UU];
+                    uint32_t keys[UU];
 #pragma unroll
                     for (int u = 0; u < UU; ++u) {
                         double qp = fabs(r1[k] - r2[u]) * cq;
@@ -269,10 +282,9 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                             qp = qp * bp64;
                             qt = qt * bt64;
                         }
-                        bpk[u] = (uint32_t)__double2int_rz(qp);
-                        btk[u] = (uint32_t)__double2int_rz(qt);
+                        keys[u] = bin_key((uint32_t)__double2int_rz(qp), (uint32_t)__double2int_rz(qt));
                     }
-                    run_sums_impl(UC, bpk, btk, w1[k], w2, d1[k], d2);
+                    run_sums_impl(UC, keys, w1[k], w2, d1[k], d2);
                 }
                 } else {
 #pragma unroll
@@ -312,10 +324,11 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                 T1[k] = valid[k] ? (float)((r1[k] + base2) * sb + (double)tau) : 1048576.5f;
             }
             const CAS float* o2p = cptr(p.off32) + p2;
+            // one LDS key per pair; the FP32 edge test is recomputed in the (rare)
+            // recheck branch instead of keeping per-pair fractions and bins in registers
             auto block = [&](int j, auto UC) {
                 constexpr int UU = decltype(UC)::value;
-                uint32_t bp[UU][PPL], bt[UU][PPL];
-                float fr[UU][PPL];
+                uint32_t key[UU][PPL];
                 float o2v[UU];
                 double w2[UU], d2[UU];
                 float frmin = 1.0f;
@@ -326,16 +339,18 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                     w2[u] = w2p[jv + u];
                     d2[u] = d2p[jv + u];
                 }
+                auto qpt = [&](int u, int k, float& qp, float& qt) {
+                    qp = __builtin_fabsf(__builtin_fmaf(o2v[u], nc4, P1[k])) + tau;
+                    qt = __builtin_fmaf(o2v[u], s4, T1[k]);
+                };
 #pragma unroll
                 for (int u = 0; u < UU; ++u) {
 #pragma unroll
                     for (int k = 0; k < PPL; ++k) {
-                        const float qp = __builtin_fabsf(__builtin_fmaf(o2v[u], nc4, P1[k])) + tau;
-                        const float qt = __builtin_fmaf(o2v[u], s4, T1[k]);
-                        bp[u][k] = __float2uint_rz(qp);
-                        bt[u][k] = __float2uint_rz(qt);
-                        fr[u][k] = fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt));
-                        frmin = fminf(frmin, fr[u][k]);
+                        float qp, qt;
+                        qpt(u, k, qp, qt);
+                        key[u][k] = bin_key(__float2uint_rz(qp), __float2uint_rz(qt));
+                        frmin = fminf(frmin, fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt)));
                     }
                 }
                 if (frmin < tau2) {
@@ -343,10 +358,12 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                     for (int u = 0; u < UU; ++u) {
 #pragma unroll
                         for (int k = 0; k < PPL; ++k) {
-                            if (fr[u][k] < tau2) {
+                            float qp, qt;
+                            qpt(u, k, qp, qt);
+                            if (fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt)) < tau2) {
                                 const double r2 = r2p[jv + u];
-                                bp[u][k] = (uint32_t)__double2int_rz((fabs(r1[k] - r2) * c) * bp64);
-                                bt[u][k] = (uint32_t)__double2int_rz(((r1[k] + r2) * s) * bt64);
+                                key[u][k] = bin_key((uint32_t)__double2int_rz((fabs(r1[k] - r2) * c) * bp64),
+                                                    (uint32_t)__double2int_rz(((r1[k] + r2) * s) * bt64));
                             }
                         }
                     }
@@ -354,19 +371,16 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                 if constexpr (MERGE) {
 #pragma unroll
                     for (int k = 0; k < PPL; ++k) {
-                        uint32_t bpk[UU], btk[UU];
+                        uint32_t keys[UU];
 #pragma unroll
-                        for (int u = 0; u < UU; ++u) {
-                            bpk[u] = bp[u][k];
-                            btk[u] = bt[u][k];
-                        }
-                        run_sums_impl(UC, bpk, btk, w1[k], w2, d1[k], d2);
+                        for (int u = 0; u < UU; ++u) keys[u] = key[u][k];
+                        run_sums_impl(UC, keys, w1[k], w2, d1[k], d2);
                     }
                 } else {
 #pragma unroll
                     for (int u = 0; u < UU; ++u) {
 #pragma unroll
-                        for (int k = 0; k < PPL; ++k) accumulate(bp[u][k], bt[u][k], w1[k] * w2[u], d1[k] * d2[u]);
+                        for (int k = 0; k < PPL; ++k) accumulate_key(key[u][k], w1[k] * w2[u], d1[k] * d2[u]);
                     }
                 }
             };
