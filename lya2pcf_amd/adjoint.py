@@ -38,7 +38,13 @@ class AdjointEngine:
         for k in ("we", "dw", "we32", "dw32"):
             self.bwd.d[k] = self.fwd.d[k]
         self.mode = mode
+        # adjoint walks: mode and workgroup size; forward sums: the tuned tile shape of the
+        # mode, without per-bin counts (the gradient needs only W and DW)
         self.cfg = _xi.default_config(mode)
+        from . import autotune
+        shape = dict(autotune.load_best(mode, walk=0) or {})
+        shape.pop("walk", None)
+        self.cfg_fwd = _xi.default_config(mode, counts=False, **shape)
         self.nb = np_rp * np_rt
 
     def set_inputs(self, dw, w):
@@ -49,7 +55,7 @@ class AdjointEngine:
         d["we32"].copy_(w.detach().float())
 
     def sums(self):
-        self.fwd.run(self.cfg)
+        self.fwd.run(self.cfg_fwd)
         return self.fwd.out_w.sum(0), self.fwd.out_dw.sum(0)
 
     def adjoint(self, alpha, beta):

@@ -1070,7 +1070,7 @@ __global__ void __launch_bounds__(WAVES * 64) xi_adjoint_kernel(const lya_xi_pro
                     const CAS double* r2p = cptr(p.dc) + p2;
                     auto block = [&](int j, auto UC) {
                         constexpr int UU = decltype(UC)::value;
-                        float o2[UU], fr[UU];
+                        float o2[UU];
                         double d2[UU], w2[UU];
                         uint32_t bp[UU], bt[UU];
                         float frmin = 1.0f;
@@ -1080,19 +1080,25 @@ __global__ void __launch_bounds__(WAVES * 64) xi_adjoint_kernel(const lya_xi_pro
                             d2[u] = d2p[j + u];
                             w2[u] = w2p[j + u];
                         }
+                        auto qpt = [&](int u, float& qp, float& qt) {
+                            qp = __builtin_fabsf(__builtin_fmaf(o2[u], -c4, P1)) + tau;
+                            qt = __builtin_fmaf(o2[u], s4, T1);
+                        };
 #pragma unroll
                         for (int u = 0; u < UU; ++u) {
-                            const float qp = __builtin_fabsf(__builtin_fmaf(o2[u], -c4, P1)) + tau;
-                            const float qt = __builtin_fmaf(o2[u], s4, T1);
+                            float qp, qt;
+                            qpt(u, qp, qt);
                             bp[u] = __float2uint_rz(qp);
                             bt[u] = __float2uint_rz(qt);
-                            fr[u] = fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt));
-                            frmin = fminf(frmin, fr[u]);
+                            frmin = fminf(frmin, fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt)));
                         }
+                        // the edge test is recomputed here rather than kept per pair
                         if (frmin < tau2) {
 #pragma unroll
                             for (int u = 0; u < UU; ++u) {
-                                if (fr[u] < tau2) {
+                                float qp, qt;
+                                qpt(u, qp, qt);
+                                if (fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt)) < tau2) {
                                     const double r2 = r2p[j + u];
                                     bp[u] = (uint32_t)__double2int_rz((fabs(r1 - r2) * c) * bp64);
                                     bt[u] = (uint32_t)__double2int_rz(((r1 + r2) * s) * bt64);
