@@ -114,8 +114,9 @@ def bench_c4(args):
     D, wA = eng.result()
     td = float(np.median(times))
     log("distortion: %d subsampled forest pairs, %.4g pixel pairs, %.3f s" % (len(pairs), P, td))
-    # chi^2 gradient: forward pair sums + two adjoint walks
-    ae = adjoint.AdjointEngine(fs, am, mode="mixed", pairs=(eng.nb_off, eng.nb_idx))
+    # chi^2 gradient: forward pair sums + two adjoint walks, FP64 geometry (on MI355X it
+    # runs at the FP32 rate and needs no edge recheck; both modes give the same bins)
+    ae = adjoint.AdjointEngine(fs, am, mode="fp64", pairs=(eng.nb_off, eng.nb_idx))
     dev = "cuda"
     wm = adjoint.WeightModel(s.z, s.sigma2_pip, cfg.z_nodes, dev)
     pr = adjoint.Projection(s.pix_off, s.loglam, dev)
@@ -144,6 +145,7 @@ def bench_c4(args):
                       "pixel_pairs_per_s": P / td,
                       "rows_with_weight": int((wA > 0).sum()),
                       "chi2_gradient": {"seconds": tgm, "pixel_pairs": P2, "pixel_pairs_per_s": 3 * P2 / tgm,
+                                        "mode": "fp64",
                                         "note": "one forward and two adjoint walks of the full pair list"}}),
           flush=True)
 

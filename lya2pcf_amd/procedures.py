@@ -15,8 +15,8 @@ Extra keyword arguments of ``init`` (all optional):
   compute_pixels  pixels that will be asked for (default: every pixel of data)
   precision       'fp64' (reference formula, default) or 'mixed'
   config          an XiConfig (default: the tuned shape of the precision)
-  counts          also count pixel pairs per bin (default True; off saves one
-                  LDS atomic per pair)
+  counts          also count pixel pairs per bin (default False, as the reference,
+                  whose outputs are the two sums; on costs one LDS atomic per run)
   rpmax, rtmax    default from the ``parameters`` module
 Pair counts per bin are available from ``pair_counts(pixel)``.
 """
@@ -37,7 +37,7 @@ def _params():
 
 
 def init(data_aux, log_file_aux, shape_hist_aux, angmax_aux, pixel_list=None, compute_pixels=None,
-         precision="fp64", rpmax=None, rtmax=None, config=None, counts=True):
+         precision="fp64", rpmax=None, rtmax=None, config=None, counts=False):
     dr, dt = _params()
     _S.clear()
     _S["log"] = log_file_aux
@@ -89,7 +89,7 @@ def two_point_per_pixel(pixel, **kargs):
 
 def pair_counts(pixel):
     """Number of pixel pairs per bin (uint64) for HEALPix ``pixel``."""
-    if not _S.get("counts", True):
+    if not _S.get("counts", False):
         raise RuntimeError("pair counts were not requested (init(..., counts=False))")
     _compute()
     k = _S["slot_of"].get(pixel)
