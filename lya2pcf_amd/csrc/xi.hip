@@ -100,13 +100,15 @@ __device__ __forceinline__ void lds_add(unsigned char* lds, uint32_t byte_off, T
 
 // LDS layout (bytes).  MIXED_F32: per replica [cnt u32 | w f32 | dw f32] x HS,
 // then the FP64 accumulators [w f64 | dw f64] x HS.  FP64 and MIXED: per
-// replica [cnt u32 | w f64 | dw f64] x HS.
-template <int MODE>
+// replica [cnt u32 | w f64 | dw f64] x HS, without the count array when pairs
+// are not counted (less LDS to clear and flush per tile).
+template <int MODE, bool CNT>
 struct Layout {
     static constexpr bool F32 = (MODE == LYA_MODE_MIXED_F32);
-    static constexpr uint32_t REPL = F32 ? 12u * HS : 20u * HS;
-    static constexpr uint32_t W = 4u * HS;                   // w offset in a replica
-    static constexpr uint32_t DW = F32 ? 8u * HS : 12u * HS;  // dw offset
+    static constexpr bool HAS_CNT = F32 || CNT;
+    static constexpr uint32_t REPL = F32 ? 12u * HS : (CNT ? 20u : 16u) * HS;
+    static constexpr uint32_t W = HAS_CNT ? 4u * HS : 0u;      // w offset in a replica
+    static constexpr uint32_t DW = F32 ? 8u * HS : W + 8u * HS;  // dw offset
     static constexpr __host__ __device__ uint32_t bytes(int R) { return REPL * (uint32_t)R + (F32 ? 16u * HS : 0u); }
     static constexpr __host__ __device__ uint32_t acc(int R) { return REPL * (uint32_t)R; }
 };
@@ -155,7 +157,7 @@ template <int MODE, int PPL, bool CNT, bool POW2, bool MERGE>
 __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char* lds, uint32_t hbase,
                                          uint32_t ntpad4, const TaskCtx& t, int slab, int f2, int lane, int perm,
                                          double bp64, double bt64) {
-    using L = Layout<MODE>;
+    using L = Layout<MODE, CNT>;
     const int NP = p.np_rp, NT = p.np_rt;
     const int64_t p2 = cptr(p.pix_off)[f2];
     const int n2 = (int)(cptr(p.pix_off)[f2 + 1] - p2);
@@ -478,7 +480,7 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
 
 // Mixed mode: move the FP32 partial sums into the FP64 accumulators.
 __device__ void flush_item(unsigned char* lds, int R, int nb_lds, int tid, int nthr) {
-    using L = Layout<LYA_MODE_MIXED_F32>;
+    using L = Layout<LYA_MODE_MIXED_F32, true>;
     double* aw = (double*)(lds + L::acc(R));
     double* ad = aw + HS;
     for (int b = tid; b < nb_lds; b += nthr) {
@@ -500,16 +502,18 @@ __device__ void flush_item(unsigned char* lds, int R, int nb_lds, int tid, int n
 template <int MODE, bool CNT>
 __device__ void flush_tile(const lya_xi_problem& p, unsigned char* lds, int R, int ntpad, int slot, int tid,
                            int nthr) {
-    using L = Layout<MODE>;
+    using L = Layout<MODE, CNT>;
     const int nb_lds = p.np_rp * ntpad;
     const int64_t gofs = (int64_t)slot * p.np_rp * p.np_rt;
     for (int b = tid; b < nb_lds; b += nthr) {
         uint32_t cnt = 0;
         double sw = 0.0, sd = 0.0;
         for (int r = 0; r < R; ++r) {
-            uint32_t* hc = (uint32_t*)(lds + r * L::REPL);
-            cnt += hc[b];
-            hc[b] = 0u;
+            if constexpr (L::HAS_CNT) {
+                uint32_t* hc = (uint32_t*)(lds + r * L::REPL);
+                cnt += hc[b];
+                hc[b] = 0u;
+            }
             if constexpr (!L::F32) {
                 double* hw = (double*)(lds + r * L::REPL + L::W);
                 double* hd = (double*)(lds + r * L::REPL + L::DW);
@@ -527,8 +531,9 @@ __device__ void flush_tile(const lya_xi_problem& p, unsigned char* lds, int R, i
             aw[b] = 0.0;
             ad[b] = 0.0;
         }
-        if (CNT ? (cnt != 0u) : (sw != 0.0 || sd != 0.0)) {
-            const int bp = b / ntpad, bt = b - bp * ntpad;
+        const int bp = b / ntpad, bt = b - bp * ntpad;
+        // the padding column bt = np_rt never holds a pair
+        if (bt < p.np_rt && (CNT ? (cnt != 0u) : (sw != 0.0 || sd != 0.0))) {
             const int64_t g = gofs + (int64_t)bp * p.np_rt + bt;
             if (CNT) atomicAdd((unsigned long long*)(p.out_cnt + g), (unsigned long long)cnt);
             unsafeAtomicAdd(p.out_w + g, sw);
@@ -539,7 +544,7 @@ __device__ void flush_tile(const lya_xi_problem& p, unsigned char* lds, int R, i
 
 template <int MODE, int PPL, int WAVES, bool CNT, bool POW2, int R, bool MERGE>
 __global__ void __launch_bounds__(WAVES * 64) xi_kernel(const lya_xi_problem p, int ntpad) {
-    using L = Layout<MODE>;
+    using L = Layout<MODE, CNT>;
     // statically sized, so histogram addresses fold into the ds offset field
     __shared__ __align__(16) unsigned char lds[L::bytes(R)];
     __shared__ int s_tile;
@@ -1255,8 +1260,9 @@ int resolve(const lya_xi_problem* p, const lya_xi_config* cfg, Resolved& r) {
     } else {
         r.ntpad = 0;
     }
-    r.lds = 16 + ((r.mode == LYA_MODE_MIXED_F32) ? Layout<LYA_MODE_MIXED_F32>::bytes(r.R)
-                                                 : Layout<LYA_MODE_FP64>::bytes(r.R));  // static LDS
+    r.lds = 16 + ((r.mode == LYA_MODE_MIXED_F32) ? Layout<LYA_MODE_MIXED_F32, true>::bytes(r.R)
+                  : cnt ? Layout<LYA_MODE_FP64, true>::bytes(r.R)
+                        : Layout<LYA_MODE_FP64, false>::bytes(r.R));  // static LDS
     if (walk) r.lds = 16 + WalkLayout::BYTES + WALK_SG * 8 + WALK_NB * (8 * 5 + 4 * 5);
     int dev = 0;
     hipDeviceProp_t prop;
