@@ -977,12 +977,13 @@ __global__ void __launch_bounds__(WAVES * 64) xi_walk_kernel(const lya_xi_proble
 // Adjoint: g_dw[i] += alpha[A] dw[j], g_w[i] += beta[A] w[j] over the pairs of
 // the list, lanes = forest-1 pixels (no atomics in the pair loop).
 // ---------------------------------------------------------------------------
-template <int MODE, int WAVES>
+template <int MODE, int WAVES, int PPL>
 __global__ void __launch_bounds__(WAVES * 64) xi_adjoint_kernel(const lya_xi_problem p, const double* alpha,
                                                                 const double* beta, double* g_dw, double* g_w) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int s_tile;
     constexpr int NTHR = WAVES * 64;
+    constexpr int SL = 64 * PPL;   // forest-1 pixels of a task: lane l holds l, l + 64, ...
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = uni(tid >> 6);
@@ -1011,7 +1012,7 @@ __global__ void __launch_bounds__(WAVES * 64) xi_adjoint_kernel(const lya_xi_pro
             const int nb1 = cptr(p.items)[4 * it + 2];
             const int64_t p1 = cptr(p.pix_off)[f1];
             const int n1 = (int)(cptr(p.pix_off)[f1 + 1] - p1);
-            const int nslab = (n1 + 63) / 64;
+            const int nslab = (n1 + SL - 1) / SL;
             const int ntask = nslab * (nb1 - nb0);
             for (int k = wave; k < ntask; k += WAVES) {
                 const int q = k / nslab;
@@ -1024,20 +1025,27 @@ __global__ void __launch_bounds__(WAVES * 64) xi_adjoint_kernel(const lya_xi_pro
                            cptr(p.uz)[f2], c, s);
                 c = unid(c);
                 s = unid(s);
-                const int i = slab * 64 + lane;
-                const bool valid = i < n1;
-                const int ilast = min(n1, slab * 64 + 64) - 1;
+                const int ilast = min(n1, slab * SL + SL) - 1;
                 int jlo, jhi;
-                prune(p, f1, p1, slab * 64, ilast, f2, p2, n2, c, s, bp64, bt64, lane, jlo, jhi);
-                if (jlo >= jhi || !valid) continue;
-                const double r1 = p.dc[p1 + i];
-                double gd = 0.0, gw = 0.0;
+                prune(p, f1, p1, slab * SL, ilast, f2, p2, n2, c, s, bp64, bt64, lane, jlo, jhi);
+                if (jlo >= jhi) continue;
+                int i[PPL];
+                bool valid[PPL];
+                double r1[PPL], gd[PPL], gw[PPL];
+#pragma unroll
+                for (int m = 0; m < PPL; ++m) {
+                    i[m] = slab * SL + m * 64 + lane;
+                    valid[m] = i[m] < n1;
+                    r1[m] = valid[m] ? p.dc[p1 + i[m]] : 1.0e9;   // far away: no pair in range
+                    gd[m] = 0.0;
+                    gw[m] = 0.0;
+                }
                 // in-range pairs gather alpha, beta of their bin; others read bin 0 times zero
-                auto gather = [&](uint32_t bp, uint32_t bt, double dwj, double wj) {
+                auto gather = [&](int m, uint32_t bp, uint32_t bt, double dwj, double wj) {
                     const bool ok = bp < (uint32_t)NP && bt < (uint32_t)NT;
                     const int A = ok ? (int)(bp * NT + bt) : 0;
-                    gd += ta[A] * (ok ? dwj : 0.0);
-                    gw += tb[A] * (ok ? wj : 0.0);
+                    gd[m] = fma(ta[A], ok ? dwj : 0.0, gd[m]);
+                    gw[m] = fma(tb[A], ok ? wj : 0.0, gw[m]);
                 };
                 // forest-2 pixels in blocks of U wave-uniform scalar loads, as in the forward walk
                 constexpr int U = 8;
@@ -1056,9 +1064,12 @@ __global__ void __launch_bounds__(WAVES * 64) xi_adjoint_kernel(const lya_xi_pro
                         }
 #pragma unroll
                         for (int u = 0; u < UU; ++u) {
-                            const uint32_t bp = (uint32_t)__double2int_rz((fabs(r1 - r2[u]) * c) * bp64);
-                            const uint32_t bt = (uint32_t)__double2int_rz(((r1 + r2[u]) * s) * bt64);
-                            gather(bp, bt, d2[u], w2[u]);
+#pragma unroll
+                            for (int m = 0; m < PPL; ++m) {
+                                const uint32_t bp = (uint32_t)__double2int_rz((fabs(r1[m] - r2[u]) * c) * bp64);
+                                const uint32_t bt = (uint32_t)__double2int_rz(((r1[m] + r2[u]) * s) * bt64);
+                                gather(m, bp, bt, d2[u], w2[u]);
+                            }
                         }
                     };
                     int j = jlo;
@@ -1069,15 +1080,19 @@ __global__ void __launch_bounds__(WAVES * 64) xi_adjoint_kernel(const lya_xi_pro
                     const double cb = c * bp64, sb = s * bt64;
                     const float c4 = unif((float)cb), s4 = unif((float)sb);
                     const float tau = p.tau, tau2 = 2.0f * tau;
-                    const float P1 = (float)((r1 - base2) * cb);
-                    const float T1 = (float)((r1 + base2) * sb + (double)tau);
+                    float P1[PPL], T1[PPL];
+#pragma unroll
+                    for (int m = 0; m < PPL; ++m) {
+                        P1[m] = valid[m] ? (float)((r1[m] - base2) * cb) : 1048576.5f;  // 2^20 + 1/2: out of range
+                        T1[m] = valid[m] ? (float)((r1[m] + base2) * sb + (double)tau) : 1048576.5f;
+                    }
                     const CAS float* o2p = cptr(p.off32) + p2;
                     const CAS double* r2p = cptr(p.dc) + p2;
                     auto block = [&](int j, auto UC) {
                         constexpr int UU = decltype(UC)::value;
                         float o2[UU];
                         double d2[UU], w2[UU];
-                        uint32_t bp[UU], bt[UU];
+                        uint32_t bp[UU][PPL], bt[UU][PPL];
                         float frmin = 1.0f;
 #pragma unroll
                         for (int u = 0; u < UU; ++u) {
@@ -1085,40 +1100,53 @@ __global__ void __launch_bounds__(WAVES * 64) xi_adjoint_kernel(const lya_xi_pro
                             d2[u] = d2p[j + u];
                             w2[u] = w2p[j + u];
                         }
-                        auto qpt = [&](int u, float& qp, float& qt) {
-                            qp = __builtin_fabsf(__builtin_fmaf(o2[u], -c4, P1)) + tau;
-                            qt = __builtin_fmaf(o2[u], s4, T1);
+                        auto qpt = [&](int u, int m, float& qp, float& qt) {
+                            qp = __builtin_fabsf(__builtin_fmaf(o2[u], -c4, P1[m])) + tau;
+                            qt = __builtin_fmaf(o2[u], s4, T1[m]);
                         };
 #pragma unroll
                         for (int u = 0; u < UU; ++u) {
-                            float qp, qt;
-                            qpt(u, qp, qt);
-                            bp[u] = __float2uint_rz(qp);
-                            bt[u] = __float2uint_rz(qt);
-                            frmin = fminf(frmin, fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt)));
+#pragma unroll
+                            for (int m = 0; m < PPL; ++m) {
+                                float qp, qt;
+                                qpt(u, m, qp, qt);
+                                bp[u][m] = __float2uint_rz(qp);
+                                bt[u][m] = __float2uint_rz(qt);
+                                frmin = fminf(frmin, fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt)));
+                            }
                         }
                         // the edge test is recomputed here rather than kept per pair
                         if (frmin < tau2) {
 #pragma unroll
                             for (int u = 0; u < UU; ++u) {
-                                float qp, qt;
-                                qpt(u, qp, qt);
-                                if (fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt)) < tau2) {
-                                    const double r2 = r2p[j + u];
-                                    bp[u] = (uint32_t)__double2int_rz((fabs(r1 - r2) * c) * bp64);
-                                    bt[u] = (uint32_t)__double2int_rz(((r1 + r2) * s) * bt64);
+#pragma unroll
+                                for (int m = 0; m < PPL; ++m) {
+                                    float qp, qt;
+                                    qpt(u, m, qp, qt);
+                                    if (fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt)) < tau2) {
+                                        const double r2 = r2p[j + u];
+                                        bp[u][m] = (uint32_t)__double2int_rz((fabs(r1[m] - r2) * c) * bp64);
+                                        bt[u][m] = (uint32_t)__double2int_rz(((r1[m] + r2) * s) * bt64);
+                                    }
                                 }
                             }
                         }
 #pragma unroll
-                        for (int u = 0; u < UU; ++u) gather(bp[u], bt[u], d2[u], w2[u]);
+                        for (int u = 0; u < UU; ++u)
+#pragma unroll
+                            for (int m = 0; m < PPL; ++m) gather(m, bp[u][m], bt[u][m], d2[u], w2[u]);
                     };
                     int j = jlo;
                     for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
                     for (; j < jhi; ++j) block(j, std::integral_constant<int, 1>{});
                 }
-                unsafeAtomicAdd(g_dw + p1 + i, gd);
-                unsafeAtomicAdd(g_w + p1 + i, gw);
+#pragma unroll
+                for (int m = 0; m < PPL; ++m) {
+                    if (valid[m]) {
+                        unsafeAtomicAdd(g_dw + p1 + i[m], gd[m]);
+                        unsafeAtomicAdd(g_w + p1 + i[m], gw[m]);
+                    }
+                }
             }
         }
     }
@@ -1422,7 +1450,9 @@ extern "C" int lya_xi_adjoint(const lya_xi_problem* p, const lya_xi_config* cfg,
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
         return lya_fail(LYA_ERR_HIP, "no HIP device");
     if (lds > prop.sharedMemPerBlock) return lya_fail(LYA_ERR_UNSUPPORTED, "too many bins for the adjoint");
-    auto fn = (mode == LYA_MODE_FP64) ? xi_adjoint_kernel<LYA_MODE_FP64, 8> : xi_adjoint_kernel<LYA_MODE_MIXED, 8>;
+    // one forest-1 pixel per lane: two (128-pixel tasks) need ~160 VGPRs for the batched
+    // bin gathers, 3 waves per SIMD, and measured 38 % slower
+    auto fn = (mode == LYA_MODE_FP64) ? xi_adjoint_kernel<LYA_MODE_FP64, 8, 1> : xi_adjoint_kernel<LYA_MODE_MIXED, 8, 1>;
     hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, 512, lds) != hipSuccess || per_cu < 1)
