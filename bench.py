@@ -84,7 +84,7 @@ def cpu_baseline(fs, eng, angmax, budget_pairs, threads):
                             "sample": "%d forest-1 lines (%.3g pixel pairs, %.1f s)" % (m1, P1, dt1)}}
 
 
-def bench_c4(args):
+def bench_distortion(args):
     """Distortion matrix (5% subsample, FP64 accumulation) and one chi^2 gradient on
     20k forests: wall times of the device work (reported, not the headline)."""
     import random
@@ -92,7 +92,7 @@ def bench_c4(args):
     import torch
     from lya2pcf_amd import adjoint, forests, synth
     from lya2pcf_amd import distortion as dist
-    cfg = synth.CONFIGS["c4"]
+    cfg = synth.CONFIGS["dist20k"]
     if args.forests:
         cfg = synth.scaled(cfg, args.forests)
     s = synth.generate(cfg)
@@ -155,8 +155,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c3", "c4"],
-                    help="c1 (default): the xi headline; c0/c3: xi on those surveys; c4: distortion + gradient")
+    ap.add_argument("--config", default="sky100k", choices=["patch2k", "sky100k", "sky300k", "dist20k"],
+                    help="sky100k (default): the xi headline; patch2k/sky300k: xi on those surveys; dist20k: distortion + gradient")
     ap.add_argument("--forests", type=int, default=0, help="override the number of forests (same density)")
     ap.add_argument("--cpu-pairs", type=float, default=1.5e11, help="CPU baseline sample size in pixel pairs")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -175,8 +175,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); use --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path (xi would shard by output "
+              "HEALPix pixel with no exchange step, the distortion with one 50 MB sum-reduce); "
+              "use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -184,8 +185,8 @@ def main():
     from lya2pcf_amd import autotune, forests, synth, xi
 
     torch.cuda.init()
-    if args.config == "c4":
-        return bench_c4(args)
+    if args.config == "dist20k":
+        return bench_distortion(args)
     cfgname = args.config
     cfg = synth.CONFIGS[cfgname]
     if args.forests:
@@ -282,7 +283,7 @@ def main():
                      "frac": achieved / peak,
                      # HBM bytes per launch measured with rocprofv3 counters on the 100k-forest
                      # workload only; other workloads report none
-                     "traffic": autotune.measured_traffic(main_mode) if args.config == "c1" and
+                     "traffic": autotune.measured_traffic(main_mode) if args.config == "sky100k" and
                      not args.forests else None,
                      "note": "10 FLOP per pixel pair (algorithmic) / median per-launch hipEvent time; "
                              "no MFMA or HBM bound applies"},

@@ -18,7 +18,7 @@ without the per-bin pair counts; the best shape per workload and mode is
 written to ``tuned.json`` next to this file and used by the benchmark and the
 drop-in procedures.
 
-    python -m lya2pcf_amd.autotune [--config c1] [--forests N] [--modes mixed,fp64] [--counts]
+    python -m lya2pcf_amd.autotune [--config sky100k] [--forests N] [--modes mixed,fp64] [--counts]
 """
 import itertools
 import json
@@ -36,15 +36,15 @@ WALK_SPACE = {"walk": (1,), "waves": (8, 16)}
 SHAPE_KEYS = ("px_per_lane", "waves", "replicas", "row_pad", "lane_map", "walk", "merge")
 
 
-def load_best(mode="mixed", config="c1", walk=None):
-    """Tuned kernel shape of ``mode`` for workload ``config`` (falls back to c1);
+def load_best(mode="mixed", config="sky100k", walk=None):
+    """Tuned kernel shape of ``mode`` for workload ``config`` (falls back to sky100k);
     with ``walk`` = 0 or 1, the best shape of that kernel family."""
     try:
         with open(CACHE) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    entry = (t.get(config) or t.get("c1") or {}).get(mode, {})
+    entry = (t.get(config) or t.get("sky100k") or {}).get(mode, {})
     if walk is None:
         best = entry.get("shape")
     else:
@@ -54,7 +54,7 @@ def load_best(mode="mixed", config="c1", walk=None):
 
 def measured_traffic(mode="mixed"):
     """HBM bytes per launch from the committed rocprofv3 counter summary, if any."""
-    path = os.path.join(PROFILES, "traffic_xi_%s_c1.json" % mode)
+    path = os.path.join(PROFILES, "traffic_xi_%s_sky100k.json" % mode)
     try:
         with open(path) as f:
             return json.load(f).get("hbm_bytes_per_launch")
@@ -102,7 +102,7 @@ def main():
     sys.path.insert(0, os.path.dirname(HERE))
     from lya2pcf_amd import forests, synth, xi
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", default="c1")
+    ap.add_argument("--config", default="sky100k")
     ap.add_argument("--forests", type=int, default=30000)
     ap.add_argument("--modes", default="mixed,fp64")
     ap.add_argument("--reps", type=int, default=3)

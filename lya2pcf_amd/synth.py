@@ -59,12 +59,12 @@ class SurveyConfig:
 
 # The five workloads of the benchmark plan.
 CONFIGS = {
-    "c0": SurveyConfig("c0: 2k forests, 10x10 deg, zq U[2.1,3.0]", 2000, 99.7, 5.0, (2.1, 3.0), 1000, 32),
-    "c1": SurveyConfig("c1: 100k forests, 2500 deg2, nside 16, zq U[2.1,3.5]", 100_000, 2500.0, 25.0,
+    "patch2k": SurveyConfig("2k forests, 10x10 deg, zq U[2.1,3.0]", 2000, 99.7, 5.0, (2.1, 3.0), 1000, 32),
+    "sky100k": SurveyConfig("100k forests, 2500 deg2, nside 16, zq U[2.1,3.5]", 100_000, 2500.0, 25.0,
                        (2.1, 3.5), 1001, 16),
-    "c3": SurveyConfig("c3: 300k forests, 8000 deg2, nside 8, density 1+9U, n U{50..600}", 300_000, 8000.0,
+    "sky300k": SurveyConfig("300k forests, 8000 deg2, nside 8, density 1+9U, n U{50..600}", 300_000, 8000.0,
                        40.0, (2.1, 3.5), 1003, 8, length_range=(50, 600), density_contrast=9.0),
-    "c4": SurveyConfig("c4: 20k forests, 1000 deg2, zq U[2.1,3.0]", 20_000, 1000.0, 15.0, (2.1, 3.0), 1004, 32),
+    "dist20k": SurveyConfig("20k forests, 1000 deg2, zq U[2.1,3.0]", 20_000, 1000.0, 15.0, (2.1, 3.0), 1004, 32),
 }
 
 

@@ -35,7 +35,7 @@ def pytest_collection_modifyitems(config, items):
 def small_survey():
     """150 forests of the 10x10 deg recipe at c0 density (seed 7)."""
     from lya2pcf_amd import synth
-    return synth.generate(synth.scaled(synth.CONFIGS["c0"], 150, seed=7))
+    return synth.generate(synth.scaled(synth.CONFIGS["patch2k"], 150, seed=7))
 
 
 @pytest.fixture(scope="session")

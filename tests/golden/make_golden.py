@@ -16,7 +16,7 @@ from lya2pcf_amd import forests, synth  # noqa: E402
 
 
 def main():
-    s = synth.generate(synth.scaled(synth.CONFIGS["c0"], 24, seed=99, length_range=(20, 40)))
+    s = synth.generate(synth.scaled(synth.CONFIGS["patch2k"], 24, seed=99, length_range=(20, 40)))
     fs = forests.ForestSet.from_survey(s)
     am = forests.angmax_from(fs, 200.0)
     off, idx = oracle.pairs(fs, am)

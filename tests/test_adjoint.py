@@ -12,7 +12,7 @@ from lya2pcf_amd import adjoint, forests, synth
 
 def test_weight_model_and_projection_cpu():
     import torch
-    s = synth.generate(synth.scaled(synth.CONFIGS["c0"], 30, seed=4))
+    s = synth.generate(synth.scaled(synth.CONFIGS["patch2k"], 30, seed=4))
     wm = adjoint.WeightModel(s.z, s.sigma2_pip, s.config.z_nodes, "cpu")
     w = wm(torch.tensor(1.0, dtype=torch.float64), torch.full((10,), 0.05, dtype=torch.float64))
     np.testing.assert_allclose(w.numpy(), s.we, rtol=1e-15)
@@ -25,7 +25,7 @@ def test_weight_model_and_projection_cpu():
 
 
 def _setup(n=80, seed=17):
-    s = synth.generate(synth.scaled(synth.CONFIGS["c0"], n, seed=seed, length_range=(40, 80)))
+    s = synth.generate(synth.scaled(synth.CONFIGS["patch2k"], n, seed=seed, length_range=(40, 80)))
     fs = forests.ForestSet.from_survey(s, float32_inputs=True)
     return s, fs, forests.angmax_from(fs, 200.0)
 
@@ -138,7 +138,7 @@ def test_projection_device_matches_host():
     """The device projection (HIP segment sums) against torch's own segment sums on
     the host: values and gradients w.r.t. delta and w."""
     import torch
-    s = synth.generate(synth.scaled(synth.CONFIGS["c0"], 60, seed=8))
+    s = synth.generate(synth.scaled(synth.CONFIGS["patch2k"], 60, seed=8))
     out = {}
     for dev in ("cpu", "cuda"):
         pr = adjoint.Projection(s.pix_off, s.loglam, dev)
@@ -155,7 +155,7 @@ def test_weight_model_gradient_cpu():
     """The node interpolation's hand-written backward (per-node sums over pixels
     sorted by interval) against torch autograd of the same expression."""
     import torch
-    s = synth.generate(synth.scaled(synth.CONFIGS["c0"], 40, seed=6))
+    s = synth.generate(synth.scaled(synth.CONFIGS["patch2k"], 40, seed=6))
     wm = adjoint.WeightModel(s.z, s.sigma2_pip, s.config.z_nodes, "cpu")
     g = torch.cos(torch.arange(len(s.z), dtype=torch.float64))
     v = torch.linspace(0.03, 0.08, 10, dtype=torch.float64).requires_grad_(True)

@@ -12,7 +12,7 @@ from lya2pcf_amd import forests, synth, xi
 
 
 def _short_set(n, lo, hi, seed):
-    cfg = synth.scaled(synth.CONFIGS["c0"], n, seed=seed, length_range=(lo, hi))
+    cfg = synth.scaled(synth.CONFIGS["patch2k"], n, seed=seed, length_range=(lo, hi))
     return forests.ForestSet.from_survey(synth.generate(cfg))
 
 
@@ -130,7 +130,7 @@ def test_distortion_gpu_many_active_bins():
 def test_distortion_gpu_short_and_flat_forests():
     """Forests of 2-6 pixels, every fifth with dl = 0 (L = sum w dl^2 = 0, the zero-slope
     convention of both the oracle and the kernel), and pairs without any in-range pixel pair."""
-    cfg = synth.scaled(synth.CONFIGS["c0"], 200, seed=31, length_range=(1, 6))
+    cfg = synth.scaled(synth.CONFIGS["patch2k"], 200, seed=31, length_range=(1, 6))
     fs = forests.ForestSet.from_survey(synth.generate(cfg))
     for f in range(0, fs.nf, 5):
         fs.dl[fs.pix_off[f]:fs.pix_off[f + 1]] = 0.0

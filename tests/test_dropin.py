@@ -25,7 +25,7 @@ def _workspace(tmp_path, n=150, seed=7, **kw):
     par = par.replace("corr_dir = './outputs/test/'", "corr_dir = '%s/out/'" % tmp_path)
     (tmp_path / "parameters.py").write_text(par)
     (tmp_path / "deltas").mkdir()
-    s = synth.generate(synth.scaled(synth.CONFIGS["c0"], n, seed=seed, **kw))   # nside 32, as parameters.py
+    s = synth.generate(synth.scaled(synth.CONFIGS["patch2k"], n, seed=seed, **kw))   # nside 32, as parameters.py
     synth.write_data_npy(s, str(tmp_path / "deltas" / "data1.npy"))
     return s
 
@@ -40,7 +40,7 @@ def _run(tmp_path, *args):
 
 def test_data_npy_round_trip(tmp_path):
     """data1.npy (pickled {pixel: [quasar]}) packs to the same arrays as the survey."""
-    s = synth.generate(synth.scaled(synth.CONFIGS["c0"], 40, seed=3))
+    s = synth.generate(synth.scaled(synth.CONFIGS["patch2k"], 40, seed=3))
     path = tmp_path / "data1.npy"
     synth.write_data_npy(s, str(path))
     data = np.load(path, allow_pickle=True).item()
@@ -63,7 +63,7 @@ def test_data_npy_round_trip(tmp_path):
 def test_quasar_neighborhood_matches_pair_list():
     """forest_class.quasar.neighborhood (healpy-free) gives the native pair list."""
     from lya2pcf_amd import xi
-    s = synth.generate(synth.scaled(synth.CONFIGS["c0"], 120, seed=5))
+    s = synth.generate(synth.scaled(synth.CONFIGS["patch2k"], 120, seed=5))
     data = synth.to_data_dict(s)
     fs = forests.ForestSet.from_data(data)
     am = forests.angmax_from(fs, 200.0)
@@ -123,7 +123,7 @@ def test_distortion_driver(tmp_path):
 def _two_files(tmp_path):
     """data1.npy and data2.npy: two surveys over the same patch (shared pixels)."""
     s1 = _workspace(tmp_path, n=80, seed=21, length_range=(30, 80))
-    s2 = synth.generate(synth.scaled(synth.CONFIGS["c0"], 70, seed=22, length_range=(30, 80)))
+    s2 = synth.generate(synth.scaled(synth.CONFIGS["patch2k"], 70, seed=22, length_range=(30, 80)))
     synth.write_data_npy(s2, str(tmp_path / "deltas" / "data2.npy"))
     for f in ("2pla_multiple_data.py", "distortion_multiple_data.py"):
         shutil.copy(os.path.join(ROOT, f), tmp_path / f)
@@ -140,7 +140,7 @@ def test_driver_helpers(tmp_path):
     assert parts[0] + parts[1] == files and len(parts[0]) == 2
     with pytest.raises(FileNotFoundError):
         drivers.data_files(str(tmp_path / "nothing"))
-    s = synth.generate(synth.scaled(synth.CONFIGS["c0"], 30, seed=4))
+    s = synth.generate(synth.scaled(synth.CONFIGS["patch2k"], 30, seed=4))
     dmin, am = drivers.cut_angle(synth.to_data_dict(s), 200.0)
     fs = forests.ForestSet.from_survey(s)
     assert dmin == fs.dc[fs.pix_off[:-1]].min() and am == forests.angmax_from(fs, 200.0)

@@ -85,7 +85,7 @@ def test_writer_round_trip(tmp_path):
 
 # ------------------------------------------------------------ delta files
 def _survey(n=40, seed=3):
-    return synth.generate(synth.scaled(synth.CONFIGS["c0"], n, seed=seed, length_range=(30, 90)))
+    return synth.generate(synth.scaled(synth.CONFIGS["patch2k"], n, seed=seed, length_range=(30, 90)))
 
 
 def _raw(s, f):

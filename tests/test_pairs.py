@@ -122,7 +122,7 @@ def test_device_pairs_match_oracle(case):
 @pytest.mark.gpu
 def test_device_pairs_and_items_match_host():
     import torch
-    s = synth.generate(synth.scaled(synth.CONFIGS["c1"], 20_000, seed=11))
+    s = synth.generate(synth.scaled(synth.CONFIGS["sky100k"], 20_000, seed=11))
     fs = forests.ForestSet.from_survey(s)
     am = forests.angmax_from(fs, 200.0)
     print("device vs host pairs", fs.nf, flush=True)

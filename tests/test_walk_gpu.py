@@ -85,7 +85,7 @@ def test_walk_edge_pairs_bit_exact():
 def test_walk_batches_and_long_forests():
     """Items of > 64 neighbours (several batches) and 600-pixel forests (a batch
     cut by the sub-group capacity) against the tile kernel."""
-    s = synth.generate(synth.scaled(synth.CONFIGS["c3"], 4000, seed=31))
+    s = synth.generate(synth.scaled(synth.CONFIGS["sky300k"], 4000, seed=31))
     fs = forests.ForestSet.from_survey(s, float32_inputs=True)
     am = forests.angmax_from(fs, 200.0)
     print("long forests", fs.nf, int(np.max(fs.n)), flush=True)
