@@ -300,22 +300,35 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
                 }
                 __syncthreads();
                 if (ch < nchunk_j && t0 == 0) {
-                    // rank rows from this chunk's eta21, eta31 rows (each thread owns its bins)
-                    const int j0 = ch * (kc_max / 2);
-                    for (int b = tid; b < nact; b += DTHREADS) {
-                        double t00 = T[b], t01 = T[npad + b], t10 = T[2 * npad + b], t11 = T[3 * npad + b];
-                        for (int jj = 0; 2 * jj < kc; ++jj) {
+                    // rank rows from this chunk's eta21, eta31 rows: the chunk's j range is split
+                    // into parts so that all threads take a share; parts add into T atomically
+                    // (T is this workgroup's own scratch)
+                    const int j0 = ch * (kc_max / 2), nj = kc / 2;
+                    const int nparts = max(1, min(nj, DTHREADS / nact));
+                    const int per = (nj + nparts - 1) / nparts;
+                    for (int e = tid; e < nparts * nact; e += DTHREADS) {
+                        const int part = e / nact, b = e - part * nact;
+                        const int jb = part * per, je = min(nj, jb + per);
+                        double t00 = 0.0, t01 = 0.0, t10 = 0.0, t11 = 0.0;
+                        for (int jj = jb; jj < je; ++jj) {
                             const double z21 = zs[(2 * jj) * npad + b], z31 = zs[(2 * jj + 1) * npad + b];
                             const double wj = sw2[j0 + jj], wdj = swd2[j0 + jj];
-                            t00 += wj * z21;
-                            t01 += wdj * z21;
-                            t10 += wj * z31;
-                            t11 += wdj * z31;
+                            t00 = fma(wj, z21, t00);
+                            t01 = fma(wdj, z21, t01);
+                            t10 = fma(wj, z31, t10);
+                            t11 = fma(wdj, z31, t11);
                         }
-                        T[b] = t00;
-                        T[npad + b] = t01;
-                        T[2 * npad + b] = t10;
-                        T[3 * npad + b] = t11;
+                        if (nparts == 1) {
+                            T[b] += t00;
+                            T[npad + b] += t01;
+                            T[2 * npad + b] += t10;
+                            T[3 * npad + b] += t11;
+                        } else {
+                            unsafeAtomicAdd(T + b, t00);
+                            unsafeAtomicAdd(T + npad + b, t01);
+                            unsafeAtomicAdd(T + 2 * npad + b, t10);
+                            unsafeAtomicAdd(T + 3 * npad + b, t11);
+                        }
                     }
                 }
                 if (ch == nchunk_j + nchunk_i && t0 == 0) {
