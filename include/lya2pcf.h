@@ -41,8 +41,9 @@ extern "C" {
 #define LYA_MODE_MIXED 1  /* FP32 geometry + FP64 near-edge recheck, FP64 products and LDS sums */
 #define LYA_MODE_MIXED_F32 2  /* FP32 geometry + recheck, FP32 products and LDS partials -> FP64 merge */
 
-/* Largest (numpix_rp * row stride) the LDS histogram holds. */
-#define LYA_MAX_BINS 2560
+/* Largest (numpix_rp * row stride) the LDS histogram holds: 50 x 51 for the default
+   binning.  At 16 bytes per entry four workgroups' histograms fit a CU's 160 KiB of LDS. */
+#define LYA_MAX_BINS 2550
 
 const char* lya_last_error(void);
 int lya_version(void);

@@ -13,7 +13,7 @@ LIB_PATH = os.environ.get("LYA2PCF_LIB") or os.path.join(os.path.dirname(os.path
 LYA_MODE_FP64 = 0
 LYA_MODE_MIXED = 1
 LYA_MODE_MIXED_F32 = 2
-LYA_MAX_BINS = 2560
+LYA_MAX_BINS = 2550
 
 _vp = C.c_void_p
 _i32 = C.c_int32

@@ -542,8 +542,22 @@ __device__ void flush_tile(const lya_xi_problem& p, unsigned char* lds, int R, i
     }
 }
 
+// Shapes that can hold 8 waves per SIMD (32 per CU) ask the compiler for that
+// occupancy: their VGPRs stay within 64 (all FP64 shapes, one pixel per lane in the
+// FP32-geometry modes) and the LDS of the 32/WAVES workgroups it takes fits the CU's
+// 160 KiB.  The compiler then keeps the SGPRs within that occupancy, spilling the rest
+// to VGPR lanes; otherwise it uses all 106, which allows 7 waves per SIMD, i.e. three
+// workgroups of 8 waves.
+template <int MODE, int PPL, bool CNT, int R, int WAVES>
+constexpr int tile_waves_per_eu() {
+    const bool vgpr_fit = MODE == LYA_MODE_FP64 || PPL == 1;
+    const bool lds_fit = WAVES >= 8 && (Layout<MODE, CNT>::bytes(R) + 4u) * (32u / WAVES) <= 160u * 1024u;
+    return vgpr_fit && lds_fit ? 8 : 1;
+}
+
 template <int MODE, int PPL, int WAVES, bool CNT, bool POW2, int R, bool MERGE>
-__global__ void __launch_bounds__(WAVES * 64) xi_kernel(const lya_xi_problem p, int ntpad) {
+__global__ void __launch_bounds__(WAVES * 64)
+    __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<MODE, PPL, CNT, R, WAVES>()))) xi_kernel(const lya_xi_problem p, int ntpad) {
     using L = Layout<MODE, CNT>;
     // statically sized, so histogram addresses fold into the ds offset field
     __shared__ __align__(16) unsigned char lds[L::bytes(R)];
