@@ -10,15 +10,19 @@ Search space (all shapes give identical bins):
     merge        0, 1      -> sum runs of equal bins per block before the LDS atomics
   walk 1 (lanes walk diagonals, FP64 only):
     waves        8, 16
-The default sweep fixes replicas=1 and row_pad=1 and skips lane_map 0 (a second
-LDS replica halves occupancy; the padding removes bank conflicts; consecutive lanes
-are 33 % slower: all measured in a full sweep).
+Two replicas exist for 8-wave workgroups only; four would not leave room in the
+CU's 160 KiB of LDS for a second workgroup.  The default sweep fixes replicas=1 and
+row_pad=1 and skips lane_map 0 and 4-wave workgroups; ``--full`` sweeps the whole
+space.  In profiles/autotune_full_sweep_sky100k.md a second LDS replica (half the
+workgroups per CU) costs 27-29 % and 4-wave workgroups 25 %, while padding and lane
+order change at most 4 % in the merged shapes; the default sweep holds the fastest
+shape.
 Each candidate is timed as hipGraph replays of the whole problem, with or
 without the per-bin pair counts; the best shape per workload and mode is
 written to ``tuned.json`` next to this file and used by the benchmark and the
 drop-in procedures.
 
-    python -m lya2pcf_amd.autotune [--config sky100k] [--forests N] [--modes mixed,fp64] [--counts]
+    python -m lya2pcf_amd.autotune [--config sky100k] [--forests N] [--modes mixed,fp64] [--counts] [--full]
 """
 import itertools
 import json
@@ -32,6 +36,8 @@ PROFILES = os.path.join(os.path.dirname(HERE), "profiles")
 
 SPACE = {"px_per_lane": (1, 2, 4), "replicas": (1,), "row_pad": (1,), "waves": (8, 16), "lane_map": (1, 2),
          "merge": (0, 1)}
+FULL_SPACE = {"px_per_lane": (1, 2, 4), "replicas": (1, 2), "row_pad": (0, 1), "waves": (4, 8, 16),
+              "lane_map": (0, 1, 2), "merge": (0, 1)}
 WALK_SPACE = {"walk": (1,), "waves": (8, 16)}
 SHAPE_KEYS = ("px_per_lane", "waves", "replicas", "row_pad", "lane_map", "walk", "merge")
 
@@ -62,22 +68,23 @@ def measured_traffic(mode="mixed"):
         return None
 
 
-def candidates(mode, space=None):
-    spaces = [space or SPACE] + ([WALK_SPACE] if mode == "fp64" and space is None else [])
+def candidates(mode, full=False):
+    """Shapes to time: the tile space (default or full) and, for fp64, the walk."""
+    spaces = [FULL_SPACE if full else SPACE] + ([WALK_SPACE] if mode == "fp64" else [])
     for sp in spaces:
         keys = list(sp)
         for vals in itertools.product(*sp.values()):
             c = dict(zip(keys, vals))
-            if c.get("replicas", 1) > c.get("waves", 8):
+            if c.get("replicas", 1) == 2 and c.get("waves", 8) != 8:  # built for 8 waves only
                 continue
             yield c
 
 
-def tune(eng, mode, reps=3, log=print, counts=False):
+def tune(eng, mode, reps=3, log=print, counts=False, full=False):
     import torch
     from . import xi
     results = []
-    for shape in candidates(mode):
+    for shape in candidates(mode, full):
         try:
             g = eng.graph(xi.default_config(mode, counts=counts, **shape))
         except Exception as e:  # shape not supported (LDS size)
@@ -107,6 +114,7 @@ def main():
     ap.add_argument("--modes", default="mixed,fp64")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--counts", action="store_true", help="tune with per-bin pair counts on")
+    ap.add_argument("--full", action="store_true", help="sweep the full shape space")
     a = ap.parse_args()
     cfg = synth.scaled(synth.CONFIGS[a.config], a.forests)
     fs = forests.ForestSet.from_survey(synth.generate(cfg), float32_inputs=True)
@@ -114,7 +122,7 @@ def main():
     print("tuning on %d forests, %.3g pixel pairs" % (fs.nf, eng.pixel_pairs), flush=True)
     out = {}
     for mode in a.modes.split(","):
-        res = tune(eng, mode, a.reps, log=lambda s: print(s, flush=True), counts=a.counts)
+        res = tune(eng, mode, a.reps, log=lambda s: print(s, flush=True), counts=a.counts, full=a.full)
         out[mode] = {"shape": res[0][1], "seconds": res[0][0], "pixel_pairs": eng.pixel_pairs,
                      "counts": a.counts, "workload": cfg.name + " (%d forests)" % fs.nf,
                      "all": [{"shape": s, "seconds": t} for t, s in res]}
