@@ -989,10 +989,13 @@ __global__ void __launch_bounds__(WAVES * 64) xi_walk_kernel(const lya_xi_proble
 
 // ---------------------------------------------------------------------------
 // Adjoint: g_dw[i] += alpha[A] dw[j], g_w[i] += beta[A] w[j] over the pairs of
-// the list, lanes = forest-1 pixels (no atomics in the pair loop).
+// the list, lanes = forest-1 pixels (no atomics in the pair loop).  Held to 80
+// VGPRs (6 waves per SIMD, three 8-wave workgroups per CU) instead of the 86-93 the
+// compiler would take (two workgroups); the few spilled values are outside the
+// gather loop.  Asking for 8 waves (64 VGPRs) spills inside it.
 // ---------------------------------------------------------------------------
 template <int MODE, int WAVES, int PPL>
-__global__ void __launch_bounds__(WAVES * 64) xi_adjoint_kernel(const lya_xi_problem p, const double* alpha,
+__global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(6))) xi_adjoint_kernel(const lya_xi_problem p, const double* alpha,
                                                                 const double* beta, double* g_dw, double* g_w) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int s_tile;
