@@ -86,3 +86,27 @@ def test_missing_library_fails_loudly(tmp_path):
                          timeout=120)
     assert out.returncode == 0, out.stderr
     assert out.stdout.startswith("raised") and "not built" in out.stdout
+
+
+def test_histogram_capacity_refused_before_any_device_call():
+    """Binnings beyond the LDS histograms are refused while the configuration is
+    resolved, before the library touches a device or the problem's buffers."""
+    import ctypes as C
+    L = _lib.lib()
+
+    def run(np_rp, np_rt, **cfg):
+        p = _lib.XiProblem()  # binning only: every buffer pointer stays NULL
+        p.np_rp, p.np_rt, p.rpmax, p.rtmax = np_rp, np_rt, 4.0 * np_rp, 4.0 * np_rt
+        return L.lya_xi_run(C.byref(p), C.byref(xi.default_config("fp64", counts=False, **cfg)), None)
+
+    # tile kernel: np_rp * (np_rt + 1) entries, at most LYA_MAX_BINS = 50 * 51
+    assert _lib.LYA_MAX_BINS == 50 * 51
+    assert run(51, 50) == _lib.LYA_ERR_UNSUPPORTED
+    assert b"2601" in L.lya_last_error()
+    assert run(50, 51) == _lib.LYA_ERR_UNSUPPORTED
+    # the default 50 x 50 binning fits exactly: refused later, for the NULL buffers
+    # (or for want of a device here), never for capacity
+    assert run(50, 50) in (_lib.LYA_ERR_ARG, _lib.LYA_ERR_HIP)
+    assert run(50, 49, row_pad=0) in (_lib.LYA_ERR_ARG, _lib.LYA_ERR_HIP)
+    # diagonal walk: (np_rp + 1) * (np_rt + 1) LDS bins, at most 2688
+    assert run(52, 51, walk=1, waves=16) == _lib.LYA_ERR_UNSUPPORTED
