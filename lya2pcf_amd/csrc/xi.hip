@@ -319,11 +319,17 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
             const float s4 = unif((float)sb);
             const float tau = p.tau;
             const float tau2 = 2.0f * tau;
-            float P1[PPL], T1[PPL];
+            // Lanes past the end of forest 1 get the coordinates 2^20 + 1/2 (out of range)
+            // and zero slopes, so that they stay exactly there: their fractions are 1/2
+            // and never call the recheck.  (With the slopes applied they would fall on
+            // the 1/8 grid of FP32 at 2^20 and hit a zero fraction every eighth pair.)
+            float P1[PPL], T1[PPL], ncl[PPL], scl[PPL];
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
-                P1[k] = valid[k] ? (float)((r1[k] - base2) * cb) : 1048576.5f;  // 2^20 + 1/2: out of range
+                P1[k] = valid[k] ? (float)((r1[k] - base2) * cb) : 1048576.5f;
                 T1[k] = valid[k] ? (float)((r1[k] + base2) * sb + (double)tau) : 1048576.5f;
+                ncl[k] = valid[k] ? nc4 : 0.0f;
+                scl[k] = valid[k] ? s4 : 0.0f;
             }
             const CAS float* o2p = cptr(p.off32) + p2;
             // one LDS key per pair; the FP32 edge test is recomputed in the (rare)
@@ -341,9 +347,12 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                     w2[u] = w2p[jv + u];
                     d2[u] = d2p[jv + u];
                 }
+                // r_par below half a bin is bin 0 in both precisions: lifting it to 1/2
+                // keeps its zero edge out of the recheck (pixels on a common wavelength
+                // grid meet at r_par = 0 exactly, once per lane and neighbour)
                 auto qpt = [&](int u, int k, float& qp, float& qt) {
-                    qp = __builtin_fabsf(__builtin_fmaf(o2v[u], nc4, P1[k])) + tau;
-                    qt = __builtin_fmaf(o2v[u], s4, T1[k]);
+                    qp = fmaxf(__builtin_fabsf(__builtin_fmaf(o2v[u], ncl[k], P1[k])) + tau, 0.5f);
+                    qt = __builtin_fmaf(o2v[u], scl[k], T1[k]);
                 };
 #pragma unroll
                 for (int u = 0; u < UU; ++u) {
@@ -400,7 +409,9 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
         const float s4 = unif((float)sb);
         const float tau = p.tau;
         const float tau2 = 2.0f * tau;
-        float P1[PPL], T1[PPL], w1[PPL], d1[PPL];
+        // lanes past the end of forest 1: fixed out-of-range coordinates 2^20 + 1/2
+        // (zero slopes keep their fractions at 1/2, clear of the recheck)
+        float P1[PPL], T1[PPL], w1[PPL], d1[PPL], ncl[PPL], scl[PPL];
 #pragma unroll
         for (int k = 0; k < PPL; ++k) {
             if (valid[k]) {
@@ -408,11 +419,15 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                 T1[k] = (float)((r1[k] + base2) * sb + (double)tau);
                 w1[k] = p.we32[pix1[k]];
                 d1[k] = p.dw32[pix1[k]];
+                ncl[k] = nc4;
+                scl[k] = s4;
             } else {
                 P1[k] = 1048576.5f;
                 T1[k] = 1048576.5f;
                 w1[k] = 0.f;
                 d1[k] = 0.f;
+                ncl[k] = 0.f;
+                scl[k] = 0.f;
             }
         }
         const CAS float* o2p = cptr(p.off32) + p2;
@@ -437,8 +452,9 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
             for (int u = 0; u < UU; ++u) {
 #pragma unroll
                 for (int k = 0; k < PPL; ++k) {
-                    const float qp = __builtin_fabsf(__builtin_fmaf(o2v[u], nc4, P1[k])) + tau;
-                    const float qt = __builtin_fmaf(o2v[u], s4, T1[k]);
+                    // r_par below half a bin: bin 0 either way, kept out of the recheck
+                    const float qp = fmaxf(__builtin_fabsf(__builtin_fmaf(o2v[u], ncl[k], P1[k])) + tau, 0.5f);
+                    const float qt = __builtin_fmaf(o2v[u], scl[k], T1[k]);
                     bp[u][k] = __float2uint_rz(qp);
                     bt[u][k] = __float2uint_rz(qt);
                     fr[u][k] = fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt));
@@ -1097,11 +1113,15 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
                     const double cb = c * bp64, sb = s * bt64;
                     const float c4 = unif((float)cb), s4 = unif((float)sb);
                     const float tau = p.tau, tau2 = 2.0f * tau;
-                    float P1[PPL], T1[PPL];
+                    // lanes past the end of forest 1: 2^20 + 1/2 with zero slopes (fraction
+                    // 1/2, out of range, never rechecked)
+                    float P1[PPL], T1[PPL], ncl[PPL], scl[PPL];
 #pragma unroll
                     for (int m = 0; m < PPL; ++m) {
-                        P1[m] = valid[m] ? (float)((r1[m] - base2) * cb) : 1048576.5f;  // 2^20 + 1/2: out of range
+                        P1[m] = valid[m] ? (float)((r1[m] - base2) * cb) : 1048576.5f;
                         T1[m] = valid[m] ? (float)((r1[m] + base2) * sb + (double)tau) : 1048576.5f;
+                        ncl[m] = valid[m] ? -c4 : 0.0f;
+                        scl[m] = valid[m] ? s4 : 0.0f;
                     }
                     const CAS float* o2p = cptr(p.off32) + p2;
                     const CAS double* r2p = cptr(p.dc) + p2;
@@ -1118,8 +1138,9 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
                             w2[u] = w2p[j + u];
                         }
                         auto qpt = [&](int u, int m, float& qp, float& qt) {
-                            qp = __builtin_fabsf(__builtin_fmaf(o2[u], -c4, P1[m])) + tau;
-                            qt = __builtin_fmaf(o2[u], s4, T1[m]);
+                            // r_par below half a bin: bin 0 either way, kept out of the recheck
+                            qp = fmaxf(__builtin_fabsf(__builtin_fmaf(o2[u], ncl[m], P1[m])) + tau, 0.5f);
+                            qt = __builtin_fmaf(o2[u], scl[m], T1[m]);
                         };
 #pragma unroll
                         for (int u = 0; u < UU; ++u) {
@@ -1431,7 +1452,11 @@ extern "C" int lya_xi_tau(const lya_xi_problem* p, double max_abs_off, double ma
     // magnitude bound, follow the mixed-mode arithmetic of run_task:
     //   rounding of P1 (or T1), of the fma result, of |v| + tau, the FP32
     //   representation of off32 and of c*binner (or s*binner), plus the FP64
-    //   roundings of the reference formula.  A factor 2 of headroom is applied.
+    //   roundings of the reference formula.  r1 - base2 and r1 - r2 are exact in
+    //   FP64 (Sterbenz).  The kernel's band is exact for an error e <= tau: a pair
+    //   whose FP32 bin differs from its FP64 bin has its shifted FP32 coordinate
+    //   within tau + e above an edge and is flagged (fract < 2 tau).  The bound
+    //   alone would do; a margin of 25 % is kept.
     if (!p || !tau || !(max_abs_off >= 0)) return lya_fail(LYA_ERR_ARG, "lya_xi_tau: bad arguments");
     const double u32 = 1.0 / 16777216.0, u64 = 1.0 / 9007199254740992.0;
     auto bound = [&](double nb, double kb) {
@@ -1439,7 +1464,7 @@ extern "C" int lya_xi_tau(const lya_xi_problem* p, double max_abs_off, double ma
         const double m = max_abs_off * kb;  // largest |off * k|
         return u32 * (2.0 * q + (q + m) + 2.0 * m) + 8.0 * u64 * (q + m + 1e4 * kb);
     };
-    const double t = 2.0 * std::max(bound(p->np_rp, max_c_binner), bound(p->np_rt, max_s_binner));
+    const double t = 1.25 * std::max(bound(p->np_rp, max_c_binner), bound(p->np_rt, max_s_binner));
     *tau = t;
     return LYA_OK;
 }

@@ -110,3 +110,58 @@ def test_histogram_capacity_refused_before_any_device_call():
     assert run(50, 49, row_pad=0) in (_lib.LYA_ERR_ARG, _lib.LYA_ERR_HIP)
     # diagonal walk: (np_rp + 1) * (np_rt + 1) LDS bins, at most 2688
     assert run(52, 51, walk=1, waves=16) == _lib.LYA_ERR_UNSUPPORTED
+
+
+def test_mixed_edge_band_flags_every_misbinned_pair():
+    """Mixed mode's FP32 arithmetic, emulated in numpy (fma through an exact FP64
+    product), against the FP64 reference bins: the FP32 error stays within the bound
+    behind tau, every pair whose FP32 bin differs is flagged (fract < 2 tau), and pairs
+    at r_par = 0 exactly (pixels on a common wavelength grid) are binned without a flag."""
+    import ctypes as C
+    rng = np.random.default_rng(11)
+    max_off, binner, n = 300.0, 0.25, 400_000
+    p = _lib.XiProblem()
+    p.np_rp = p.np_rt = 50
+    t = C.c_double()
+    _lib.check(_lib.lib().lya_xi_tau(C.byref(p), max_off, binner, binner, C.byref(t)))
+    tau = np.float32(t.value)
+    if float(tau) < t.value:
+        tau = np.nextafter(tau, np.float32(1))
+    f32 = np.float32
+    fma32 = lambda a, b, c: (a.astype(np.float64) * b.astype(np.float64) + c.astype(np.float64)).astype(f32)
+    base2 = rng.integers(3600, 5600, n).astype(np.float64)
+    off2 = rng.uniform(-max_off, max_off, n)
+    r2 = base2 + off2
+    c = np.cos(rng.uniform(0, 0.03, n))          # half-angle cosines of the 3.2 deg cut
+    s = np.sqrt(1 - c * c)
+    # forest-1 distances: random, within a few tau of an r_par edge, or equal to r2
+    rp_bins = rng.uniform(0, 53, n)
+    kind = rng.integers(0, 3, n)
+    near = kind == 1
+    rp_bins[near] = np.floor(rp_bins[near]) + rng.uniform(-3, 3, near.sum()) * float(tau)
+    sign = np.where(rng.random(n) < 0.5, 1.0, -1.0)
+    r1 = r2 + sign * np.abs(rp_bins) / (c * binner)
+    same = kind == 2
+    r1[same] = r2[same]
+    cb, sb = c * binner, s * binner
+    o2 = (r2 - base2).astype(f32)
+    # kernel: P1 = (r1 - base2) c b, T1 = (r1 + base2) s b + tau, both rounded once to FP32
+    P1 = ((r1 - base2) * cb).astype(f32)
+    T1 = ((r1 + base2) * sb + float(tau)).astype(f32)
+    qv = (np.abs(fma32(o2, -cb.astype(f32), P1)) + tau).astype(f32)
+    qp = np.maximum(qv, f32(0.5))                  # below half a bin: bin 0 either way
+    qt = fma32(o2, sb.astype(f32), T1)
+    # reference FP64 coordinates (binner a power of two: folding it in is exact)
+    qp64 = (np.abs(r1 - r2) * c) * binner
+    qt64 = ((r1 + r2) * s) * binner
+    rel = (qp64 < 53) & (qt64 < 53)
+    bound = t.value / 1.25
+    assert np.abs(qv[rel] - (qp64[rel] + float(tau))).max() <= bound
+    assert np.abs(qt[rel] - (qt64[rel] + float(tau))).max() <= bound
+    flagged = np.minimum(qp - np.floor(qp), qt - np.floor(qt)) < 2 * tau
+    wrong = (np.floor(qp) != np.floor(qp64)) | (np.floor(qt) != np.floor(qt64))
+    assert wrong[rel].sum() > 0                    # the near-edge half does hit wrong FP32 bins
+    assert not np.any(wrong & rel & ~flagged)      # and every one of them is rechecked
+    # r_par = 0 exactly: bin 0, and its zero edge calls no recheck (r_perp edges still may)
+    assert np.all(qp64[same] == 0) and not np.any(wrong[same])
+    assert np.all(qp[same] - np.floor(qp[same]) >= 2 * tau)
