@@ -48,7 +48,9 @@ def test_xi_matches_oracle(small_set, small_pairs, mode, merge):
 
 
 def test_kernel_shapes(small_set, small_pairs):
-    """Every tile size / workgroup / LDS layout gives the same counts."""
+    """Every tile size / workgroup / LDS layout gives the same counts, and the same sums
+    with counts off (those shapes are separate kernels, several built for 8 waves per
+    SIMD)."""
     am, (off, idx) = small_pairs
     c0, _, _ = oracle.xi_gpu_form(small_set, off, idx)
     eng = xi.XiEngine(small_set, am, pairs=(off, idx))
@@ -57,11 +59,16 @@ def test_kernel_shapes(small_set, small_pairs):
                                                                         [0, 1, 2], [0, 1]):
         if merge and mode == "mixed32":
             continue
-        cfg = xi.default_config(mode, px_per_lane=ppl, waves=waves, replicas=rep, row_pad=pad, lane_map=lmap,
-                                merge=merge)
-        eng.run(cfg)
-        c, _, _ = eng.results()
-        assert np.array_equal(c, c0), (mode, ppl, waves, rep, pad, lmap, merge)
+        shape = dict(px_per_lane=ppl, waves=waves, replicas=rep, row_pad=pad, lane_map=lmap, merge=merge)
+        eng.run(xi.default_config(mode, **shape))
+        c, w, d = eng.results()
+        assert np.array_equal(c, c0), (mode, shape)
+        eng.run(xi.default_config(mode, counts=False, **shape))
+        c2, w2, d2 = eng.results()
+        tol = 1e-5 if mode == "mixed32" else 1e-12
+        assert c2.sum() == 0, (mode, shape)
+        assert np.all(np.abs(w2 - w) <= tol * np.abs(w) + 1e-300), (mode, shape)
+        assert np.all(np.abs(d2 - d) <= tol * np.abs(w) + 1e-300), (mode, shape)
 
 
 def test_without_counts(small_set, small_pairs):
