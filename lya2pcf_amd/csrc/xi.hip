@@ -203,16 +203,21 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
         // Sums of the products of lane pixel k with a block's forest-2 pixels, one set
         // of LDS atomics per run of equal bins (consecutive forest-2 pixels of a lane
         // often share a bin).
-        // LDS index of bin (bp, bt), or NONE when the pair is out of range
+        // LDS index of bin (bp, bt) for an in-range pair.  Out of range: NONE without
+        // merged runs; with them a trash entry in the histogram's padding column (row
+        // lane % NP, column NT; the host gives merged kernels a padded row stride),
+        // so that every run can be flushed without testing it, and the tile flush never
+        // reads that column.  Lanes own their trash entry (two lanes share one at 50 rows).
         constexpr uint32_t NONE = 0xFFFFFFFFu;
+        const uint32_t trash = MERGE ? (uint32_t)(lane % NP) * (ntpad4 >> 2) + (uint32_t)NT : NONE;
         auto bin_key = [&](uint32_t bp, uint32_t bt) -> uint32_t {
-            return (bp < (uint32_t)NP && bt < (uint32_t)NT) ? __umul24(bp, ntpad4 >> 2) + bt : NONE;
+            return (bp < (uint32_t)NP && bt < (uint32_t)NT) ? __umul24(bp, ntpad4 >> 2) + bt : trash;
         };
         auto run_sums_impl = [&](auto UC, const uint32_t* keys, double w1k, const double* w2, double d1k,
                                  const double* d2) {
             constexpr int UU = decltype(UC)::value;
-            uint32_t pk = 0xFFFFFFFFu, an = 0u;
-            double aw = 0.0, ad = 0.0;
+            uint32_t pk = keys[0], an = 1u;
+            double aw = fma(w1k, w2[0], 0.0), ad = fma(d1k, d2[0], 0.0);
             auto flush = [&]() {
                 const uint32_t a4 = (pk << 2) + hbase;
                 if constexpr (CNT) lds_add<uint32_t>(lds, a4, an);
@@ -221,10 +226,10 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                 lds_add<double>(lds, a8 + (L::DW - L::W), ad);
             };
 #pragma unroll
-            for (int u = 0; u < UU; ++u) {
+            for (int u = 1; u < UU; ++u) {
                 const uint32_t key = keys[u];
                 if (key != pk) {
-                    if (pk != 0xFFFFFFFFu) flush();
+                    flush();
                     aw = 0.0;
                     ad = 0.0;
                     an = 0u;
@@ -234,7 +239,7 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                 ad = fma(d1k, d2[u], ad);
                 ++an;
             }
-            if (pk != 0xFFFFFFFFu) flush();
+            flush();
         };
         auto accumulate = [&](uint32_t bp, uint32_t bt, double pw, double pd) {
             if (bp < (uint32_t)NP && bt < (uint32_t)NT) {
@@ -1316,7 +1321,8 @@ int resolve(const lya_xi_problem* p, const lya_xi_config* cfg, Resolved& r) {
     if (p) {
         if (p->np_rp < 1 || p->np_rt < 1 || !(p->rpmax > 0) || !(p->rtmax > 0))
             return lya_fail(LYA_ERR_ARG, "bad binning");
-        r.ntpad = p->np_rt + (walk ? 1 : pad);
+        // merged runs park out-of-range runs in the padding column: at least one
+        r.ntpad = p->np_rt + (walk ? 1 : (merge && r.mode != LYA_MODE_MIXED_F32 ? std::max(pad, 1) : pad));
         if (walk && (int64_t)(p->np_rp + 1) * r.ntpad > WALK_HS)
             return lya_fail(LYA_ERR_UNSUPPORTED, "walk 1: (np_rp+1)*(np_rt+1)=%d exceeds %d LDS bins",
                             (p->np_rp + 1) * r.ntpad, WALK_HS);
