@@ -14,9 +14,9 @@ Two replicas exist for 8-wave workgroups only; four would not leave room in the
 CU's 160 KiB of LDS for a second workgroup.  The default sweep fixes replicas=1 and
 row_pad=1 and skips lane_map 0 and 4-wave workgroups; ``--full`` sweeps the whole
 space.  In profiles/autotune_full_sweep_sky100k.md a second LDS replica (half the
-workgroups per CU) costs 27-29 % and 4-wave workgroups 25 %, while padding and lane
-order change at most 4 % in the merged shapes; the default sweep holds the fastest
-shape.
+workgroups per CU) costs 20-22 % and 4-wave workgroups 18-20 %, while padding changes
+at most 1 % and lane order up to 8 % in the merged shapes; the default sweep holds the
+fastest shapes.
 Each candidate is timed as hipGraph replays of the whole problem, with or
 without the per-bin pair counts; the best shape per workload and mode is
 written to ``tuned.json`` next to this file and used by the benchmark and the
