@@ -108,6 +108,9 @@ def test_histogram_capacity_refused_before_any_device_call():
     # (or for want of a device here), never for capacity
     assert run(50, 50) in (_lib.LYA_ERR_ARG, _lib.LYA_ERR_HIP)
     assert run(50, 49, row_pad=0) in (_lib.LYA_ERR_ARG, _lib.LYA_ERR_HIP)
+    # merged runs always take the padded stride: 50 x 51 bins fit unpadded only
+    assert run(50, 51, row_pad=0) in (_lib.LYA_ERR_ARG, _lib.LYA_ERR_HIP)
+    assert run(50, 51, row_pad=0, merge=1) == _lib.LYA_ERR_UNSUPPORTED
     # diagonal walk: (np_rp + 1) * (np_rt + 1) LDS bins, at most 2688
     assert run(52, 51, walk=1, waves=16) == _lib.LYA_ERR_UNSUPPORTED
 
