@@ -161,3 +161,26 @@ def test_empty_problem(small_set):
     eng.run(xi.default_config("mixed"))
     c, w, d = eng.results()
     assert c.sum() == 0 and not w.any() and not d.any()
+
+
+@pytest.mark.parametrize("np_rp,np_rt,rpmax,rtmax", [(1, 1, 200.0, 200.0), (3, 5, 60.0, 100.0),
+                                                     (7, 13, 61.0, 97.0)])
+def test_small_binnings_merged(small_set, small_pairs, np_rp, np_rt, rpmax, rtmax):
+    """Merged runs on binnings of 1 to 7 rows: out-of-range runs of many lanes share the
+    few trash entries of the padding column (all 64 lanes one entry at one row), and
+    non-power-of-two binners take the unfolded FP64 path; counts bit-exact, sums 1e-12."""
+    am, (off, idx) = small_pairs
+    fs = forests.ForestSet(small_set.pix_off, small_set.slots,
+                           **{k: getattr(small_set, k) for k in small_set.FOREST + small_set.PIXEL})
+    fs.we = fs.we32.astype(np.float64)
+    fs.dw = fs.dw32.astype(np.float64)
+    eng = xi.XiEngine(fs, am, np_rp=np_rp, np_rt=np_rt, rpmax=rpmax, rtmax=rtmax, pairs=(off, idx))
+    c0, w0, d0 = oracle.xi_gpu_form(fs, off, idx, np_rp, np_rt, rpmax, rtmax)
+    _, aw, ad = oracle.xi_gpu_form(fs, off, idx, np_rp, np_rt, rpmax, rtmax, we=np.abs(fs.we), dw=np.abs(fs.dw))
+    for mode in ("fp64", "mixed"):
+        for ppl in (1, 2):
+            eng.run(xi.default_config(mode, px_per_lane=ppl, merge=1))
+            c, w, d = eng.results()
+            assert np.array_equal(c, c0), (mode, ppl, np_rp, np_rt)
+            assert np.all(np.abs(w - w0) <= 1e-12 * aw + 1e-300), (mode, ppl)
+            assert np.all(np.abs(d - d0) <= 1e-12 * ad + 1e-300), (mode, ppl)
