@@ -151,7 +151,12 @@ typedef struct lya_xi_problem {
     uint64_t* out_cnt;        /* [n_slots][np_rp*np_rt] pair counts                   */
     double* out_w;            /* [n_slots][np_rp*np_rt] sum w1*w2                     */
     double* out_dw;           /* [n_slots][np_rp*np_rt] sum dw1*dw2                   */
-    uint32_t* queue;          /* [>= 4] device scratch: work-queue counter            */
+    uint32_t* queue;          /* [>= 16] device scratch: work-queue counters          */
+    const int32_t* tile_groups; /* [n_tile_groups+1] tile offsets of the tile groups;   */
+                              /* NULL: one group.  Workgroup b takes tiles from group */
+                              /* b % n_tile_groups first (consecutive workgroups sit  */
+                              /* on different XCDs), then from the others             */
+    int32_t n_tile_groups;    /* 0 or 1: one queue over all tiles; at most 16         */
 } lya_xi_problem;
 
 /* Kernel shape, chosen by the autotuner.  Zero fields take defaults. */

@@ -32,6 +32,7 @@ class XiProblem(C.Structure):
         ("rpmax", _f64), ("rtmax", _f64), ("tau", C.c_float), ("lane_map", _i32), ("max_len", _i32),
         ("pad", _i32),
         ("out_cnt", _vp), ("out_w", _vp), ("out_dw", _vp), ("queue", _vp),
+        ("tile_groups", _vp), ("n_tile_groups", _i32),
     ]
 
 

@@ -520,6 +520,28 @@ __device__ void flush_item(unsigned char* lds, int R, int nb_lds, int tid, int n
 }
 
 // End of tile: merge the workgroup's sums into the slot's FP64 global sums.
+// Next tile for this workgroup, or -1 when none is left (one thread calls it).  With
+// tile groups (neighbouring slots, one group per XCD: consecutive workgroups sit on
+// different XCDs), workgroup b serves group b % G first and then the others, so that an
+// XCD's L2 mostly holds the forests of one part of the sky.  At most G atomics; the
+// counters of exhausted groups only grow.
+__device__ __forceinline__ int next_tile(const lya_xi_problem& p) {
+    const int G = (p.tile_groups && p.n_tile_groups > 1) ? p.n_tile_groups : 1;
+    if (G == 1) {
+        const int t = (int)atomicAdd(p.queue, 1u);
+        return t < p.n_tiles ? t : -1;
+    }
+    const int g0 = (int)(blockIdx.x % (unsigned)G);
+    for (int k = 0; k < G; ++k) {
+        const int g = g0 + k < G ? g0 + k : g0 + k - G;
+        const int b = p.tile_groups[g], e = p.tile_groups[g + 1];
+        if (b >= e) continue;
+        const int t = b + (int)atomicAdd(p.queue + g, 1u);
+        if (t < e) return t;
+    }
+    return -1;
+}
+
 template <int MODE, bool CNT>
 __device__ void flush_tile(const lya_xi_problem& p, unsigned char* lds, int R, int ntpad, int slot, int tid,
                            int nthr) {
@@ -604,10 +626,10 @@ __global__ void __launch_bounds__(WAVES * 64)
     __syncthreads();
 
     for (;;) {
-        if (tid == 0) s_tile = (int)atomicAdd(p.queue, 1u);
+        if (tid == 0) s_tile = next_tile(p);
         __syncthreads();
         const int tile = uni(s_tile);
-        if (tile >= p.n_tiles) break;
+        if (tile < 0) break;
         const int ib = cptr(p.tiles)[2 * tile], ie = cptr(p.tiles)[2 * tile + 1];
         const int slot = cptr(p.items)[4 * ib + 3];
         for (int it = ib; it < ie; ++it) {
@@ -740,10 +762,10 @@ __global__ void __launch_bounds__(WAVES * 64) xi_walk_kernel(const lya_xi_proble
     __syncthreads();
 
     for (;;) {
-        if (tid == 0) s_tile = (int)atomicAdd(p.queue, 1u);
+        if (tid == 0) s_tile = next_tile(p);
         __syncthreads();
         const int tile = uni(s_tile);
-        if (tile >= p.n_tiles) break;
+        if (tile < 0) break;
         const int ib = cptr(p.tiles)[2 * tile], ie = cptr(p.tiles)[2 * tile + 1];
         const int slot = cptr(p.items)[4 * ib + 3];
         for (int it = ib; it < ie; ++it) {
@@ -1038,11 +1060,11 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
     const double bt64 = (double)NT / p.rtmax;
 
     for (;;) {
-        if (tid == 0) s_tile = (int)atomicAdd(p.queue, 1u);
+        if (tid == 0) s_tile = next_tile(p);
         __syncthreads();
         const int tile = uni(s_tile);
         __syncthreads();
-        if (tile >= p.n_tiles) break;
+        if (tile < 0) break;
         const int ib = cptr(p.tiles)[2 * tile], ie = cptr(p.tiles)[2 * tile + 1];
         for (int it = ib; it < ie; ++it) {
             const int f1 = cptr(p.items)[4 * it + 0];
@@ -1215,7 +1237,7 @@ __global__ void zero_outputs(uint64_t* cnt, double* w, double* dw, int64_t n, ui
         if (w) w[i] = 0.0;
         if (dw) dw[i] = 0.0;
     }
-    if (blockIdx.x == 0 && threadIdx.x < 4) queue[threadIdx.x] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < 16) queue[threadIdx.x] = 0u;
 }
 
 __global__ void halfangle_kernel(const double* cos12, double* c, double* s, int n) {
@@ -1360,6 +1382,8 @@ int check_problem(const lya_xi_problem* p, int mode) {
     if (mode != LYA_MODE_FP64 && !(p->tau > 0.0f && p->tau < 0.25f))
         return lya_fail(LYA_ERR_ARG, "mixed mode needs 0 < tau < 0.25 (got %g)", (double)p->tau);
     if (p->n_tiles < 0 || p->n_items < 0 || p->n_slots < 0) return lya_fail(LYA_ERR_ARG, "negative sizes");
+    if (p->n_tile_groups < 0 || p->n_tile_groups > 16 || (p->n_tile_groups > 1 && !p->tile_groups))
+        return lya_fail(LYA_ERR_ARG, "n_tile_groups must be 0..16, with tile_groups when above 1");
     return LYA_OK;
 }
 

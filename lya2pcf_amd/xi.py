@@ -82,6 +82,35 @@ def build_tiles(items, costs, tile_cost):
     return tiles[:nt.value]
 
 
+XCDS = 8  # MI355X: 8 accelerator dies (XCDs), each with its own L2
+
+
+def group_tiles(items, tiles, item_costs, n_groups):
+    """Tiles regrouped for XCD-local work: tiles ordered by slot (neighbouring HEALPix
+    pixels, hence neighbouring forests), cut into ``n_groups`` contiguous runs of about
+    equal cost, each run ordered by decreasing cost.  Returns (tiles, offsets) with
+    offsets of length n_groups + 1; the kernels' workgroup b serves group b % n_groups
+    first, and consecutive workgroups sit on different XCDs."""
+    tiles = np.asarray(tiles, dtype=np.int32).reshape(-1, 2)
+    nt = len(tiles)
+    if n_groups <= 1 or nt == 0:
+        return tiles, np.array([0, nt], dtype=np.int32)
+    cum = np.concatenate([[0], np.cumsum(np.asarray(item_costs, dtype=np.float64))])
+    tcost = cum[tiles[:, 1]] - cum[tiles[:, 0]]
+    slot = np.asarray(items, dtype=np.int32).reshape(-1, 4)[tiles[:, 0], 3]
+    order = np.lexsort((np.arange(nt), slot))
+    c = np.cumsum(tcost[order])
+    total = c[-1] if nt else 0.0
+    cuts = np.searchsorted(c, total * np.arange(1, n_groups) / n_groups, side="left")
+    offs = np.concatenate([[0], np.minimum(cuts + 1, nt), [nt]]).astype(np.int64)
+    offs = np.maximum.accumulate(offs)
+    out = []
+    for g in range(n_groups):
+        sel = order[offs[g]:offs[g + 1]]
+        out.append(sel[np.argsort(-tcost[sel], kind="stable")])
+    return tiles[np.concatenate(out)], offs.astype(np.int32)
+
+
 def _device_vec(t, dtype, n, what):
     import torch
     if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == dtype and t.is_contiguous() and t.numel() == n):
@@ -169,6 +198,7 @@ class XiEngine:
 
     def __init__(self, forests, angmax, np_rp=50, np_rt=50, rpmax=200.0, rtmax=200.0, slots=None,
                  item_cost=2_000_000, tiles_per_group=6, device="cuda", pairs=None, transposed=False,
+                 xcd_groups=XCDS,
                  pair_builder="device"):
         import torch
         if pair_builder not in ("host", "device"):
@@ -191,6 +221,7 @@ class XiEngine:
         self.f1_list = f1_list
         self.item_cost = int(item_cost)
         self.tiles_per_group = tiles_per_group
+        self.xcd_groups = int(xcd_groups)
         self._work = {}
         self._items_host = None
         self.setup_seconds = {}
@@ -286,7 +317,7 @@ class XiEngine:
         self.out_cnt = torch.zeros((self.n_slots, nbins), dtype=torch.int64, device=self.device)
         self.out_w = torch.zeros((self.n_slots, nbins), dtype=torch.float64, device=self.device)
         self.out_dw = torch.zeros((self.n_slots, nbins), dtype=torch.float64, device=self.device)
-        self.queue = torch.zeros(4, dtype=torch.int32, device=self.device)
+        self.queue = torch.zeros(16, dtype=torch.int32, device=self.device)
         self.tau = self._tau()
 
     def _tau(self):
@@ -313,13 +344,25 @@ class XiEngine:
             else:
                 items, tiles, _ = build_work(self.forests, self.nb_off, self.nb_idx, self.f1_list, self.item_cost,
                                              tile_cost)
+                costs = self._item_costs(items)
+            tiles, offs = group_tiles(items, tiles, costs, self.xcd_groups)
             self._work[key] = (self._t(items if len(items) else np.zeros((1, 4), np.int32)),
                                self._t(tiles if len(tiles) else np.zeros((1, 2), np.int32)),
-                               len(items), len(tiles))
+                               len(items), len(tiles), self._t(offs), len(offs) - 1)
         return self._work[key]
 
+    def _item_costs(self, items):
+        """n1 * sum of n2 over each item's neighbours (host arrays)."""
+        items = np.asarray(items, dtype=np.int64).reshape(-1, 4)
+        if len(items) == 0:
+            return np.zeros(0, dtype=np.int64)
+        nlen = np.diff(np.asarray(self.forests.pix_off, dtype=np.int64))
+        nb = np.asarray(self.nb_idx, dtype=np.int64)
+        cum = np.concatenate([[0], np.cumsum(nlen[nb])])
+        return nlen[items[:, 0]] * (cum[items[:, 2]] - cum[items[:, 1]])
+
     def problem(self, cfg):
-        items, tiles, ni, nt = self.work(cfg)
+        items, tiles, ni, nt, offs, ng = self.work(cfg)
         d = self.d
         p = XiProblem()
         p.n_forest, p.n_slots = self.forests.nf, self.n_slots
@@ -332,7 +375,8 @@ class XiEngine:
         p.max_len = self.max_len
         p.pad = self.pad
         p.out_cnt, p.out_w, p.out_dw, p.queue = ptr(self.out_cnt), ptr(self.out_w), ptr(self.out_dw), ptr(self.queue)
-        self._keep = (items, tiles)
+        p.tile_groups, p.n_tile_groups = ptr(offs), ng
+        self._keep = (items, tiles, offs)
         return p
 
     def run(self, cfg=None, stream=None):

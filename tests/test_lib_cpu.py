@@ -168,3 +168,31 @@ def test_mixed_edge_band_flags_every_misbinned_pair():
     # r_par = 0 exactly: bin 0, and its zero edge calls no recheck (r_perp edges still may)
     assert np.all(qp64[same] == 0) and not np.any(wrong[same])
     assert np.all(qp[same] - np.floor(qp[same]) >= 2 * tau)
+
+
+def test_group_tiles_partition():
+    """XCD tile groups: a permutation of the tiles, contiguous slot ranges per group,
+    costs balanced, each group in decreasing cost; one group keeps the tiles as built."""
+    rng = np.random.default_rng(5)
+    n_items = 400
+    slots = np.sort(rng.integers(0, 60, n_items))
+    items = np.stack([np.arange(n_items), np.zeros(n_items), np.ones(n_items), slots], 1).astype(np.int32)
+    costs = rng.integers(1, 1000, n_items)
+    tiles = xi.build_tiles(items, costs, 3000)
+    t1, o1 = xi.group_tiles(items, tiles, costs, 1)
+    assert np.array_equal(t1, tiles) and list(o1) == [0, len(tiles)]
+    t8, o8 = xi.group_tiles(items, tiles, costs, 8)
+    assert len(o8) == 9 and o8[0] == 0 and o8[-1] == len(tiles) and np.all(np.diff(o8) >= 0)
+    key = lambda t: sorted(map(tuple, t.tolist()))
+    assert key(t8) == key(tiles)
+    cum = np.concatenate([[0], np.cumsum(costs)])
+    tc = cum[t8[:, 1]] - cum[t8[:, 0]]
+    sl = items[t8[:, 0], 3]
+    g_cost = [tc[o8[g]:o8[g + 1]].sum() for g in range(8)]
+    assert max(g_cost) <= sum(g_cost) / 8 + tc.max()
+    for g in range(7):
+        a, b = sl[o8[g]:o8[g + 1]], sl[o8[g + 1]:o8[g + 2]]
+        if len(a) and len(b):
+            assert a.max() <= b.min()
+    for g in range(8):
+        assert np.all(np.diff(tc[o8[g]:o8[g + 1]]) <= 0)
