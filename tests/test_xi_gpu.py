@@ -184,3 +184,17 @@ def test_small_binnings_merged(small_set, small_pairs, np_rp, np_rt, rpmax, rtma
             assert np.array_equal(c, c0), (mode, ppl, np_rp, np_rt)
             assert np.all(np.abs(w - w0) <= 1e-12 * aw + 1e-300), (mode, ppl)
             assert np.all(np.abs(d - d0) <= 1e-12 * ad + 1e-300), (mode, ppl)
+
+
+@pytest.mark.parametrize("xcd_groups", [1, 3, 16])
+def test_tile_group_counts(small_set, small_pairs, xcd_groups):
+    """One queue (what callers get with n_tile_groups = 0), and tile groups of any count,
+    including groups left empty: every tile is taken exactly once (counts bit-exact)."""
+    am, (off, idx) = small_pairs
+    c0, _, _ = oracle.xi_gpu_form(small_set, off, idx)
+    eng = xi.XiEngine(small_set, am, pairs=(off, idx), xcd_groups=xcd_groups, item_cost=20_000)
+    for mode, shape in (("fp64", dict(px_per_lane=2, merge=1)), ("mixed", dict(merge=1)),
+                        ("fp64", dict(walk=1, waves=16))):
+        eng.run(xi.default_config(mode, **shape))
+        c, _, _ = eng.results()
+        assert np.array_equal(c, c0), (xcd_groups, mode, shape)
