@@ -1032,13 +1032,18 @@ __global__ void __launch_bounds__(WAVES * 64) xi_walk_kernel(const lya_xi_proble
 
 // ---------------------------------------------------------------------------
 // Adjoint: g_dw[i] += alpha[A] dw[j], g_w[i] += beta[A] w[j] over the pairs of
-// the list, lanes = forest-1 pixels (no atomics in the pair loop).  Held to 80
-// VGPRs (6 waves per SIMD, three 8-wave workgroups per CU) instead of the 86-93 the
-// compiler would take (two workgroups); the few spilled values are outside the
-// gather loop.  Asking for 8 waves (64 VGPRs) spills inside it.
+// the list, lanes = forest-1 pixels (no atomics in the pair loop).
+//   LDS holds (alpha, beta) interleaved, one 16-byte read per pair, plus a zero
+//   entry at index nb that out-of-range pairs read (adding 0 * w leaves the sums
+//   unchanged, so no select of the operands is needed).
+//   POW2: FP64 mode with power-of-two binners folded into c and s (exact, as in
+//   the forward kernel).
+//   At 56-62 VGPRs and (nb + 1) * 16 bytes of LDS it holds 8 waves per SIMD (four
+//   8-wave workgroups per CU at 50 x 50 bins); the compiler keeps the SGPRs within
+//   that occupancy.
 // ---------------------------------------------------------------------------
-template <int MODE, int WAVES, int PPL>
-__global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(6))) xi_adjoint_kernel(const lya_xi_problem p, const double* alpha,
+template <int MODE, int WAVES, int PPL, bool POW2>
+__global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(8))) xi_adjoint_kernel(const lya_xi_problem p, const double* alpha,
                                                                 const double* beta, double* g_dw, double* g_w) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int s_tile;
@@ -1049,12 +1054,8 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
     const int wave = uni(tid >> 6);
     const int NP = p.np_rp, NT = p.np_rt;
     const int nb = NP * NT;
-    double* ta = (double*)lds;
-    double* tb = ta + nb;
-    for (int b = tid; b < nb; b += NTHR) {
-        ta[b] = alpha[b];
-        tb[b] = beta[b];
-    }
+    double2* tab = (double2*)lds;   // [nb + 1] (alpha, beta); entry nb is zero
+    for (int b = tid; b <= nb; b += NTHR) tab[b] = b < nb ? make_double2(alpha[b], beta[b]) : make_double2(0.0, 0.0);
     __syncthreads();
     const double bp64 = (double)NP / p.rpmax;
     const double bt64 = (double)NT / p.rtmax;
@@ -1102,10 +1103,11 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
                 }
                 // in-range pairs gather alpha, beta of their bin; others read bin 0 times zero
                 auto gather = [&](int m, uint32_t bp, uint32_t bt, double dwj, double wj) {
-                    const bool ok = bp < (uint32_t)NP && bt < (uint32_t)NT;
-                    const int A = ok ? (int)(bp * NT + bt) : 0;
-                    gd[m] = fma(ta[A], ok ? dwj : 0.0, gd[m]);
-                    gw[m] = fma(tb[A], ok ? wj : 0.0, gw[m]);
+                    const uint32_t A = (bp < (uint32_t)NP && bt < (uint32_t)NT) ? __umul24(bp, (uint32_t)NT) + bt
+                                                                              : (uint32_t)nb;
+                    const double2 ab = tab[A];
+                    gd[m] = fma(ab.x, dwj, gd[m]);
+                    gw[m] = fma(ab.y, wj, gw[m]);
                 };
                 // forest-2 pixels in blocks of U wave-uniform scalar loads, as in the forward walk
                 constexpr int U = 8;
@@ -1113,6 +1115,7 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
                 const CAS double* w2p = cptr(p.we) + p2;
                 if constexpr (MODE == LYA_MODE_FP64) {
                     const CAS double* r2p = cptr(p.dc) + p2;
+                    const double cq = POW2 ? c * bp64 : c, sq = POW2 ? s * bt64 : s;
                     auto block = [&](int j, auto UC) {
                         constexpr int UU = decltype(UC)::value;
                         double r2[UU], d2[UU], w2[UU];
@@ -1126,9 +1129,9 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
                         for (int u = 0; u < UU; ++u) {
 #pragma unroll
                             for (int m = 0; m < PPL; ++m) {
-                                const uint32_t bp = (uint32_t)__double2int_rz((fabs(r1[m] - r2[u]) * c) * bp64);
-                                const uint32_t bt = (uint32_t)__double2int_rz(((r1[m] + r2[u]) * s) * bt64);
-                                gather(m, bp, bt, d2[u], w2[u]);
+                                const double qp = POW2 ? fabs(r1[m] - r2[u]) * cq : (fabs(r1[m] - r2[u]) * c) * bp64;
+                                const double qt = POW2 ? (r1[m] + r2[u]) * sq : ((r1[m] + r2[u]) * s) * bt64;
+                                gather(m, (uint32_t)__double2int_rz(qp), (uint32_t)__double2int_rz(qt), d2[u], w2[u]);
                             }
                         }
                     };
@@ -1516,7 +1519,7 @@ extern "C" int lya_xi_adjoint(const lya_xi_problem* p, const lya_xi_config* cfg,
     if (rc) return rc;
     if (!alpha || !beta || !g_dw || !g_w) return lya_fail(LYA_ERR_ARG, "null adjoint buffer");
     const int nb = p->np_rp * p->np_rt;
-    const size_t lds = (size_t)nb * 16;
+    const size_t lds = (size_t)(nb + 1) * 16;
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
@@ -1524,7 +1527,10 @@ extern "C" int lya_xi_adjoint(const lya_xi_problem* p, const lya_xi_config* cfg,
     if (lds > prop.sharedMemPerBlock) return lya_fail(LYA_ERR_UNSUPPORTED, "too many bins for the adjoint");
     // one forest-1 pixel per lane: two (128-pixel tasks) need ~160 VGPRs for the batched
     // bin gathers, 3 waves per SIMD, and measured 38 % slower
-    auto fn = (mode == LYA_MODE_FP64) ? xi_adjoint_kernel<LYA_MODE_FP64, 8, 1> : xi_adjoint_kernel<LYA_MODE_MIXED, 8, 1>;
+    const bool pow2 = is_pow2((double)p->np_rp / p->rpmax) && is_pow2((double)p->np_rt / p->rtmax);
+    auto fn = (mode == LYA_MODE_FP64) ? (pow2 ? xi_adjoint_kernel<LYA_MODE_FP64, 8, 1, true>
+                                              : xi_adjoint_kernel<LYA_MODE_FP64, 8, 1, false>)
+                                      : xi_adjoint_kernel<LYA_MODE_MIXED, 8, 1, false>;
     hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, 512, lds) != hipSuccess || per_cu < 1)
