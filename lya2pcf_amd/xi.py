@@ -335,9 +335,13 @@ class XiEngine:
     def work(self, cfg):
         groups = C.c_int32(0)
         check(lib().lya_xi_grid(C.byref(cfg), C.byref(groups)))
-        key = groups.value
+        # tile kernels: XCD tile groups (fewer L2 misses, same time); the walk, with a
+        # quarter of the workgroups, keeps one queue in global cost order (groups cost it
+        # 3.5 % at 300k forests)
+        n_groups = 1 if getattr(cfg, "walk", 0) else self.xcd_groups
+        key = (groups.value, n_groups)
         if key not in self._work:
-            tile_cost = max(self.item_cost, int(self.pixel_pairs / max(1, key * self.tiles_per_group)))
+            tile_cost = max(self.item_cost, int(self.pixel_pairs / max(1, groups.value * self.tiles_per_group)))
             if self._items_host is not None:
                 items, costs = self._items_host
                 tiles = build_tiles(items, costs, tile_cost)
@@ -345,7 +349,7 @@ class XiEngine:
                 items, tiles, _ = build_work(self.forests, self.nb_off, self.nb_idx, self.f1_list, self.item_cost,
                                              tile_cost)
                 costs = self._item_costs(items)
-            tiles, offs = group_tiles(items, tiles, costs, self.xcd_groups)
+            tiles, offs = group_tiles(items, tiles, costs, n_groups)
             self._work[key] = (self._t(items if len(items) else np.zeros((1, 4), np.int32)),
                                self._t(tiles if len(tiles) else np.zeros((1, 2), np.int32)),
                                len(items), len(tiles), self._t(offs), len(offs) - 1)
