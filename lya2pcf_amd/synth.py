@@ -257,7 +257,8 @@ def write_data_npy(survey, path):
     np.save(path, to_data_dict(survey), allow_pickle=True)
 
 
-def adversarial(n_pairs, seed=1002, n_pix=8, eps=1e-6, rmax=200.0, nbins=50, omega_m=0.315, nside=16):
+def adversarial(n_pairs, seed=1002, n_pix=8, eps=1e-6, rmax=200.0, nbins=50, omega_m=0.315, nside=16,
+                theta=(5e-4, 0.04)):
     """Forest pairs whose pixel pairs sit within ``eps`` Mpc/h of bin edges.
 
     Each pair (f1, f2) gets a random small separation theta; forest 1 has
@@ -267,7 +268,10 @@ def adversarial(n_pairs, seed=1002, n_pix=8, eps=1e-6, rmax=200.0, nbins=50, ome
     with e = 0, i.e. on the edge up to FP64 rounding).  The half-angle factors
     are computed from the stored unit vectors with the FP64 formula the
     kernels use, so the targets are edges of the evaluated coordinates.
-    Pairs sit at random sky positions, far apart from each other.
+    Pairs sit at random sky positions, far apart from each other, with
+    separations theta ~ U(theta[0], theta[1]) rad.  r_perp edges at small theta
+    need r1 + r2 = edge / s: the default range reaches ~8e5 Mpc/h; (0.03, 0.055)
+    keeps every distance below 10^4 Mpc/h, as in real data.
     """
     rng = np.random.default_rng(seed)
     bsz = rmax / nbins
@@ -277,7 +281,7 @@ def adversarial(n_pairs, seed=1002, n_pix=8, eps=1e-6, rmax=200.0, nbins=50, ome
     for k in range(n_pairs):
         ra1 = rng.uniform(0.2, 2 * np.pi - 0.2)
         dec1 = np.arcsin(rng.uniform(-0.9, 0.9))
-        th = rng.uniform(5e-4, 0.04)
+        th = rng.uniform(theta[0], theta[1])
         # forest 2 at angle th, displaced mostly in ra so that ra2 > ra1
         phi = rng.uniform(-0.6, 0.6)
         dec2 = dec1 + th * np.sin(phi)

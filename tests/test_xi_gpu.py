@@ -86,10 +86,14 @@ def test_without_counts(small_set, small_pairs):
         assert np.all(np.abs(d - d0) <= tol * np.abs(w0) + 1e-300)
 
 
-def test_adversarial_edges_bit_exact():
+@pytest.mark.parametrize("seed,theta", [(1002, (5e-4, 0.04)), (1003, (0.03, 0.055))])
+def test_adversarial_edges_bit_exact(seed, theta):
     """10,000 seeded forest pairs whose pixel pairs sit within 1e-6 Mpc/h of bin edges
-    (80,000 targeted pixel pairs): mixed bins equal FP64 bins, in every kernel."""
-    s = synth.adversarial(10_000, seed=1002)
+    (80,000 targeted pixel pairs): mixed bins equal FP64 bins, in every kernel.  The
+    first set puts r_perp edges at small angles (dc up to ~8e5 Mpc/h); the second keeps
+    every dc below 10^4 Mpc/h, as real forests are."""
+    s = synth.adversarial(10_000, seed=seed, theta=theta)
+    print("adversarial seed", seed, "theta", theta, flush=True)
     fs = forests.ForestSet.from_survey(s, float32_inputs=True)
     am = 0.06
     off, idx = xi.build_pairs(fs, am)
