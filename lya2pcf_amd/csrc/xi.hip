@@ -200,46 +200,54 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
         const CAS double* r2p = cptr(p.dc) + p2;
         const CAS double* w2p = cptr(p.we) + p2;
         const CAS double* d2p = cptr(p.dw) + p2;
-        // Sums of the products of lane pixel k with a block's forest-2 pixels, one set
-        // of LDS atomics per run of equal bins (consecutive forest-2 pixels of a lane
-        // often share a bin).
+        // Merged kernels sum the products of lane pixel k over runs of equal bins
+        // (consecutive forest-2 pixels of a lane often share a bin) and make one set of
+        // LDS atomics per run.  The open run is carried from block to block of forest-2
+        // pixels and flushed when its bin changes and once at the end of the task.
         // LDS index of bin (bp, bt) for an in-range pair.  Out of range: NONE without
         // merged runs; with them a trash entry in the histogram's padding column (row
         // lane % NP, column NT; the host gives merged kernels a padded row stride),
         // so that every run can be flushed without testing it, and the tile flush never
-        // reads that column.  Lanes own their trash entry (two lanes share one at 50 rows).
+        // reads that column.  Lanes own their trash entry (two lanes share one at 50 rows);
+        // a lane whose pairs are all out of range writes it once per task.
         constexpr uint32_t NONE = 0xFFFFFFFFu;
         const uint32_t trash = MERGE ? (uint32_t)(lane % NP) * (ntpad4 >> 2) + (uint32_t)NT : NONE;
-        auto bin_key = [&](uint32_t bp, uint32_t bt) -> uint32_t {
-            return (bp < (uint32_t)NP && bt < (uint32_t)NT) ? __umul24(bp, ntpad4 >> 2) + bt : trash;
-        };
-        auto run_sums_impl = [&](auto UC, const uint32_t* keys, double w1k, const double* w2, double d1k,
-                                 const double* d2) {
-            constexpr int UU = decltype(UC)::value;
-            uint32_t pk = keys[0], an = 1u;
-            double aw = fma(w1k, w2[0], 0.0), ad = fma(d1k, d2[0], 0.0);
-            auto flush = [&]() {
-                const uint32_t a4 = (pk << 2) + hbase;
-                if constexpr (CNT) lds_add<uint32_t>(lds, a4, an);
-                const uint32_t a8 = (a4 << 1) + woff;
-                lds_add<double>(lds, a8, aw);
-                lds_add<double>(lds, a8 + (L::DW - L::W), ad);
-            };
+        uint32_t cpk[PPL], can[PPL];
+        double caw[PPL], cad[PPL];
 #pragma unroll
-            for (int u = 1; u < UU; ++u) {
+        for (int k = 0; k < PPL; ++k) {
+            cpk[k] = trash;
+            can[k] = 0u;
+            caw[k] = 0.0;
+            cad[k] = 0.0;
+        }
+        auto flush_run = [&](uint32_t pk, double aw, double ad, uint32_t an) {
+            const uint32_t a4 = (pk << 2) + hbase;
+            if constexpr (CNT) lds_add<uint32_t>(lds, a4, an);
+            const uint32_t a8 = (a4 << 1) + woff;
+            lds_add<double>(lds, a8, aw);
+            lds_add<double>(lds, a8 + (L::DW - L::W), ad);
+        };
+        auto run_sums_carry = [&](auto UC, const uint32_t* keys, double w1k, const double* w2, double d1k,
+                                  const double* d2, uint32_t& pk, double& aw, double& ad, uint32_t& an) {
+            constexpr int UU = decltype(UC)::value;
+#pragma unroll
+            for (int u = 0; u < UU; ++u) {
                 const uint32_t key = keys[u];
                 if (key != pk) {
-                    flush();
+                    flush_run(pk, aw, ad, an);
                     aw = 0.0;
                     ad = 0.0;
                     an = 0u;
                 }
                 pk = key;
-                aw = fma(w1k, w2[u], aw);   // run sums: fused, like any FP64 partial sum here
+                aw = fma(w1k, w2[u], aw);
                 ad = fma(d1k, d2[u], ad);
                 ++an;
             }
-            flush();
+        };
+        auto bin_key = [&](uint32_t bp, uint32_t bt) -> uint32_t {
+            return (bp < (uint32_t)NP && bt < (uint32_t)NT) ? __umul24(bp, ntpad4 >> 2) + bt : trash;
         };
         auto accumulate = [&](uint32_t bp, uint32_t bt, double pw, double pd) {
             if (bp < (uint32_t)NP && bt < (uint32_t)NT) {
@@ -291,7 +299,7 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                         }
                         keys[u] = bin_key((uint32_t)__double2int_rz(qp), (uint32_t)__double2int_rz(qt));
                     }
-                    run_sums_impl(UC, keys, w1[k], w2, d1[k], d2);
+                    run_sums_carry(UC, keys, w1[k], w2, d1[k], d2, cpk[k], caw[k], cad[k], can[k]);
                 }
                 } else {
 #pragma unroll
@@ -315,6 +323,10 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
             int j = jlo;
             for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
             for (; j < jhi; ++j) block(j, std::integral_constant<int, 1>{});
+            if constexpr (MERGE) {
+#pragma unroll
+                for (int k = 0; k < PPL; ++k) flush_run(cpk[k], caw[k], cad[k], can[k]);
+            }
         } else {
             // FP32 geometry, FP64 recheck near edges, exact FP64 sums
             const double base2 = cptr(p.base)[f2];
@@ -390,7 +402,7 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                         uint32_t keys[UU];
 #pragma unroll
                         for (int u = 0; u < UU; ++u) keys[u] = key[u][k];
-                        run_sums_impl(UC, keys, w1[k], w2, d1[k], d2);
+                        run_sums_carry(UC, keys, w1[k], w2, d1[k], d2, cpk[k], caw[k], cad[k], can[k]);
                     }
                 } else {
 #pragma unroll
@@ -404,6 +416,10 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
             int j = jlo;
             for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
             for (; j < jhi; ++j) block(j, std::integral_constant<int, 1>{});
+            if constexpr (MERGE) {
+#pragma unroll
+                for (int k = 0; k < PPL; ++k) flush_run(cpk[k], caw[k], cad[k], can[k]);
+            }
         }
     } else {
         // FP32 geometry, FP32 products and FP32 LDS partials
