@@ -38,11 +38,14 @@ extern "C" {
 #define LYA_ERR_UNSUPPORTED (-4)
 
 #define LYA_MODE_FP64 0   /* plain FP64 pair loop, FP64 LDS histograms */
-#define LYA_MODE_MIXED 1  /* FP32 geometry + FP64 near-edge recheck, FP64 products and LDS sums */
-#define LYA_MODE_MIXED_F32 2  /* FP32 geometry + recheck, FP32 products and LDS partials -> FP64 merge */
+#define LYA_MODE_MIXED 1  /* FP32 geometry (FP32 dc offsets) + FP64 near-edge recheck; w, dw read as
+                             FP64, products and LDS sums in FP64 */
+#define LYA_MODE_MIXED_F32 2  /* FP32 geometry + recheck, FP32 w, dw, products and LDS partials
+                                 -> FP64 merge */
 
-/* Largest (numpix_rp * row stride) the LDS histogram holds: 50 x 51 for the default
-   binning.  At 16 bytes per entry four workgroups' histograms fit a CU's 160 KiB of LDS. */
+/* Entries (numpix_rp * row stride) of the LDS histogram of one workgroup: 50 x 51 for the
+   default binning.  At 16 bytes per entry four workgroups' histograms fit a CU's 160 KiB of
+   LDS.  Larger binnings run in bands of bins that fit it (lya_xi_run). */
 #define LYA_MAX_BINS 2550
 
 const char* lya_last_error(void);
@@ -180,7 +183,13 @@ typedef struct lya_xi_config {
 int lya_xi_tau(const lya_xi_problem* host_view_of_sizes, double max_abs_off, double max_c_binner,
                double max_s_binner, double* tau);
 
-/* Zero outputs and queue, then launch the persistent kernel on `stream`. */
+/* Zero outputs and queue, then launch the persistent kernel on `stream`.
+ * Any np_rp x np_rt binning: when np_rp * (np_rt + pad) exceeds LYA_MAX_BINS the
+ * tile kernels (fp64 and mixed modes, 8 waves, one replica) walk each tile once per
+ * band of bins that fits the LDS histogram (column bands of whole rows, or blocks),
+ * with each task pruned to the pixels that can land in the band.  mixed32, other
+ * workgroup shapes and walk 1 (at most (np_rp+1)(np_rt+1) <= 2688) return
+ * LYA_ERR_UNSUPPORTED for such binnings. */
 int lya_xi_run(const lya_xi_problem* p, const lya_xi_config* cfg, void* stream);
 
 /* The same sequence captured into a hipGraph (zeroing uses fill kernels). */
@@ -198,6 +207,8 @@ int lya_xi_grid(const lya_xi_config* cfg, int32_t* groups);
  * f2) in bin A:  g_dw[i] += alpha[A] * dw[j],  g_w[i] += beta[A] * w[j]
  * (alpha, beta: [np_rp*np_rt] FP64, global bins).  Gradients w.r.t. the
  * forest-2 side come from a second call with the transposed pair list.
+ * Any binning: (alpha, beta) sit in LDS when (nb + 1) * 16 bytes fit a workgroup,
+ * else they are read from global memory.
  * Uses the mixed geometry (exact bins) when cfg->mode is LYA_MODE_MIXED.
  * g_dw, g_w: [n_pix] FP64, accumulated (caller zeroes).
  */
@@ -220,6 +231,13 @@ int lya_segment_sum(const double* x, const int64_t* off, int32_t nseg, double* o
  * dl = delta_lambda, omega = sum w, odl2 = sum w dl^2 per forest (FP64).
  * max_len must be at least the length of every forest in the pairs (it sizes the
  * LDS staging); pairs with a longer forest are skipped.
+ * Any binning with nb = np_rp*np_rt <= 16384.  The LDS holds the Z rows of a pair's
+ * active bins in chunks; a pair with more active bins than two rows of the LDS
+ * chunk buffer hold is skipped (and counted).
+ * Skipped pairs are counted in the uint32 words LYA_DIST_SKIP_LONG (forest longer
+ * than max_len) and LYA_DIST_SKIP_BINS (too many active bins) of scratch: the
+ * counts add up over calls; the caller zeroes scratch before the first call and
+ * reads them back after the last.
  * D: [nb][nb] FP64 with nb = np_rp*np_rt, wA: [nb]; both accumulated.
  * Each pair's contribution is symmetric, and only the upper triangle of D
  * (row <= column) is accumulated: call lya_distortion_mirror() after the last
@@ -228,6 +246,8 @@ int lya_segment_sum(const double* x, const int64_t* off, int32_t nseg, double* o
  * groups: persistent workgroups (one per CU fills the GPU).
  * Replaces the kernels launched at distortion_procedures_pycuda.py:249-279.
  */
+#define LYA_DIST_SKIP_LONG 1
+#define LYA_DIST_SKIP_BINS 2
 int64_t lya_distortion_scratch_bytes(int32_t groups, int32_t max_len, int32_t nbins);
 int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, const double* dc, const double* we,
                        const double* dl, const double* ux, const double* uy, const double* uz,

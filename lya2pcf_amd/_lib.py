@@ -14,6 +14,8 @@ LYA_MODE_FP64 = 0
 LYA_MODE_MIXED = 1
 LYA_MODE_MIXED_F32 = 2
 LYA_MAX_BINS = 2550
+LYA_DIST_SKIP_LONG = 1   # uint32 words of the distortion scratch counting skipped pairs
+LYA_DIST_SKIP_BINS = 2
 
 _vp = C.c_void_p
 _i32 = C.c_int32

@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
         if (n1 > a.max_len || n2 > a.max_len) {
             // longer than the staging arrays: skipped, and counted in queue[1]; the barrier
             // keeps thread 0 from drawing the next pair before every wave has read this one
-            if (tid == 0) atomicAdd(a.queue + 1, 1u);
+            if (tid == 0) atomicAdd(a.queue + LYA_DIST_SKIP_LONG, 1u);
             __syncthreads();
             continue;
         }
@@ -208,6 +208,12 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
         __syncthreads();
         const int nact = uni(s_nact);
         if (nact == 0) continue;
+        // the LDS chunk buffer must hold two Z rows of the pair's active bins (padded to
+        // the block edge); a pair with more active bins is skipped and counted in queue[2]
+        if ((a.zcap / (((nact + BS - 1) / BS) * BS)) < 2) {
+            if (tid == 0) atomicAdd(a.queue + LYA_DIST_SKIP_BINS, 1u);
+            continue;
+        }
         // compact bin of pixel pair (i, j), or -1
         auto cbin = [&](int i, int j) -> int {
             const int g = gbin(i, j);
@@ -382,8 +388,9 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
     }
 }
 
+// Only the work counter: the skip counters add up over runs until the caller clears them.
 __global__ void zero_queue(uint32_t* q) {
-    if (threadIdx.x < 4) q[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) q[0] = 0u;
 }
 
 // D[c][r] = D[r][c] for r < c (one thread per element of the lower triangle's rows).
@@ -416,6 +423,7 @@ extern "C" int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, cons
     const int NB = np_rp * np_rt;
     if (NB > 16384) return lya_fail(LYA_ERR_UNSUPPORTED, "distortion: at most 16384 bins");
     if (max_len < 1) return lya_fail(LYA_ERR_ARG, "distortion: max_len < 1");
+    if ((int64_t)max_len > INT32_MAX / 48) return lya_fail(LYA_ERR_UNSUPPORTED, "distortion: max_len too large");
     if (groups < 1) return lya_fail(LYA_ERR_ARG, "distortion: groups < 1");
     const int64_t need = lya_distortion_scratch_bytes(groups, max_len, NB);
     if (scratch_bytes < need)
@@ -425,12 +433,13 @@ extern "C" int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, cons
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
         return lya_fail(LYA_ERR_HIP, "no HIP device");
-    // the chunk buffer takes the rest of the workgroup's LDS; it must hold two rows of
-    // every active bin (padded to the block edge)
+    // the chunk buffer takes the rest of the workgroup's LDS.  It must hold two rows of
+    // the active bins of a pair (padded to the block edge): at least one block here,
+    // and the kernel skips (and counts) a pair with more active bins than it holds
     const size_t fixed = lds_fixed(NB, max_len) + LDS_STATIC;
     const int64_t npad_max = (int64_t)(NB + BS - 1) / BS * BS;
     const size_t budget = prop.sharedMemPerBlock;
-    if (fixed + (size_t)16 * npad_max > budget)
+    if (fixed + (size_t)16 * BS > budget)
         return lya_fail(LYA_ERR_UNSUPPORTED, "distortion: %d bins and forests of %d pixels exceed the LDS", NB,
                         max_len);
     const int64_t zcap = (int64_t)((budget - fixed) / 8);

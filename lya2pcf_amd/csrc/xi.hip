@@ -19,6 +19,12 @@
 //   * histograms are privatised in LDS (one copy per workgroup, optionally
 //     replicated) and merged into per-slot FP64 global sums with atomics at the
 //     end of each tile.
+//   * binnings larger than the LDS histogram are cut into bands (blocks of
+//     r_perp columns, and of r_parallel rows when one column per row no longer
+//     fits).  A tile is walked once per band; each task's j range is pruned to
+//     the pixels that can land in the band (for sorted forests r_perp grows with
+//     j, so a forest pair mostly touches one or two column bands), and the band's
+//     sums are flushed before the next band starts.
 //
 // Precision modes:
 //   LYA_MODE_FP64   the reference formula in FP64 (no FMA contraction),
@@ -120,26 +126,56 @@ struct TaskCtx {
     int slot;
 };
 
+// Bins held by the LDS histogram: rows [bp0, bp0 + rows) x columns [bt0, bt0 + cols)
+// of the binning (the whole binning when it fits).
+struct Band {
+    uint32_t bp0, bt0, rows, cols;
+};
+
+// Band grid of a binning too large for the LDS histogram: br x bc bins per band,
+// nbp x nbt bands (a kernel argument; unused when the binning fits).
+struct BandShape {
+    int32_t br, bc, nbp, nbt;
+};
+
 // Forest-2 j range that can hold an in-range pair for some forest-1 pixel of the
 // slab.  Needs both forests sorted by dc; the margins (1e-3 bin) dominate the
 // FP64 rounding of the thresholds, so skipped pairs are out of range exactly.
+// BANDED: pairs that land in the band [bp0, bp0 + rows) x [bt0, bt0 + cols) only.
+// r_perp = (r1 + r2) s grows with j, so the band's columns give a lower bound on j
+// as well; the row bound uses the band's last row (rows from 0 upwards, a superset).
+template <bool BANDED = false>
 __device__ __forceinline__ void prune(const lya_xi_problem& p, int f1, int64_t p1, int i0, int ilast, int f2,
                                       int64_t p2, int n2, double c, double s, double bp64, double bt64, int lane,
-                                      int& jlo, int& jhi) {
+                                      int& jlo, int& jhi, const Band& bd = Band{0u, 0u, 0u, 0u}) {
     jlo = 0;
     jhi = n2;
     if (!(cptr(p.sorted)[f1] & cptr(p.sorted)[f2])) return;
     const double r1min = cptr(p.dc)[p1 + i0];
     const double r1max = cptr(p.dc)[p1 + ilast];
-    const double Dp = ((double)p.np_rp + 1e-3) / (c * bp64);
-    const double Rt = ((double)p.np_rt + 1e-3) / (s * bt64);
+    const double np_hi = BANDED ? (double)(bd.bp0 + bd.rows) : (double)p.np_rp;
+    const double nt_hi = BANDED ? (double)(bd.bt0 + bd.cols) : (double)p.np_rt;
+    const double Dp = (np_hi + 1e-3) / (c * bp64);
+    const double Rt = (nt_hi + 1e-3) / (s * bt64);
     const double lo = r1min - Dp;
     const double hi = fmin(r1max + Dp, Rt - r1min);
+    // band: r2 below this bound puts r_perp under the band's first column for every lane
+    const double lot = BANDED && bd.bt0 > 0u ? ((double)bd.bt0 - 1e-3) / (s * bt64) - r1max : -INFINITY;
+    if constexpr (BANDED) {
+        // the forest pair misses the band altogether: nothing to prune
+        if (n2 == 0 || cptr(p.dc)[p2 + n2 - 1] < lot || cptr(p.dc)[p2] >= hi) {
+            jlo = jhi = 0;
+            return;
+        }
+    }
     int clo = 0, chi = 0;
     const int nr = (n2 + 63) & ~63;
     for (int jj = lane; jj < nr; jj += 64) {
         const double r2 = (jj < n2) ? p.dc[p2 + jj] : INFINITY;
-        clo += __popcll(__ballot(r2 <= lo));
+        if constexpr (BANDED)
+            clo += __popcll(__ballot(r2 <= lo || r2 < lot));
+        else
+            clo += __popcll(__ballot(r2 <= lo));
         chi += __popcll(__ballot(r2 < hi));
     }
     jlo = uni(clo);
@@ -152,13 +188,17 @@ __device__ __forceinline__ void prune(const lya_xi_problem& p, int f1, int64_t p
 //   POW2  FP64 mode with power-of-two binners: the binner is folded into the
 //         half-angle factors, fl(|d| (c b)) = fl(|d| c) b exactly when b = 2^k,
 //         so the bin equals the reference's floor(fl(fl(|d| c) b)).
+//   BANDED  the LDS histogram holds the band bd of the binning: bins are taken
+//         relative to (bd.bp0, bd.bt0) and pairs outside the band are out of range.
 // ---------------------------------------------------------------------------
-template <int MODE, int PPL, bool CNT, bool POW2, bool MERGE>
+template <int MODE, int PPL, bool CNT, bool POW2, bool MERGE, bool BANDED>
 __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char* lds, uint32_t hbase,
                                          uint32_t ntpad4, const TaskCtx& t, int slab, int f2, int lane, int perm,
-                                         double bp64, double bt64) {
+                                         double bp64, double bt64, const Band& bd) {
+    static_assert(!(BANDED && MODE == LYA_MODE_MIXED_F32), "mixed32 has no banded form");
     using L = Layout<MODE, CNT>;
-    const int NP = p.np_rp, NT = p.np_rt;
+    // rows and columns of the LDS histogram
+    const int NP = BANDED ? (int)bd.rows : p.np_rp, NT = BANDED ? (int)bd.cols : p.np_rt;
     const int64_t p2 = cptr(p.pix_off)[f2];
     const int n2 = (int)(cptr(p.pix_off)[f2 + 1] - p2);
     double c, s;
@@ -169,7 +209,7 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
     const int i0 = slab * 64 * PPL;
     const int ilast = min(t.n1, i0 + 64 * PPL) - 1;
     int jlo, jhi;
-    prune(p, t.f1, t.p1, i0, ilast, f2, p2, n2, c, s, bp64, bt64, lane, jlo, jhi);
+    prune<BANDED>(p, t.f1, t.p1, i0, ilast, f2, p2, n2, c, s, bp64, bt64, lane, jlo, jhi, bd);
     if (jlo >= jhi) return;
 
     double r1[PPL];
@@ -247,9 +287,17 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
             }
         };
         auto bin_key = [&](uint32_t bp, uint32_t bt) -> uint32_t {
+            if constexpr (BANDED) {
+                bp -= bd.bp0;   // bins below the band wrap to large values: out of range
+                bt -= bd.bt0;
+            }
             return (bp < (uint32_t)NP && bt < (uint32_t)NT) ? __umul24(bp, ntpad4 >> 2) + bt : trash;
         };
         auto accumulate = [&](uint32_t bp, uint32_t bt, double pw, double pd) {
+            if constexpr (BANDED) {
+                bp -= bd.bp0;
+                bt -= bd.bt0;
+            }
             if (bp < (uint32_t)NP && bt < (uint32_t)NT) {
                 if constexpr (CNT) {
                     const uint32_t a4 = __umul24(bp, ntpad4) + opaque((bt << 2) + hbase);
@@ -560,10 +608,11 @@ __device__ __forceinline__ int next_tile(const lya_xi_problem& p) {
 
 template <int MODE, bool CNT>
 __device__ void flush_tile(const lya_xi_problem& p, unsigned char* lds, int R, int ntpad, int slot, int tid,
-                           int nthr) {
+                           int nthr, const Band& bd) {
     using L = Layout<MODE, CNT>;
-    const int nb_lds = p.np_rp * ntpad;
-    const int64_t gofs = (int64_t)slot * p.np_rp * p.np_rt;
+    const int nb_lds = (int)bd.rows * ntpad;
+    // global bin of LDS entry (bp, bt): (bd.bp0 + bp) np_rt + bd.bt0 + bt
+    const int64_t gofs = (int64_t)slot * p.np_rp * p.np_rt + (int64_t)bd.bp0 * p.np_rt + bd.bt0;
     for (int b = tid; b < nb_lds; b += nthr) {
         uint32_t cnt = 0;
         double sw = 0.0, sd = 0.0;
@@ -591,8 +640,8 @@ __device__ void flush_tile(const lya_xi_problem& p, unsigned char* lds, int R, i
             ad[b] = 0.0;
         }
         const int bp = b / ntpad, bt = b - bp * ntpad;
-        // the padding column bt = np_rt never holds a pair
-        if (bt < p.np_rt && (CNT ? (cnt != 0u) : (sw != 0.0 || sd != 0.0))) {
+        // columns from bd.cols on (padding, trash entries) never hold a pair
+        if (bt < (int)bd.cols && (CNT ? (cnt != 0u) : (sw != 0.0 || sd != 0.0))) {
             const int64_t g = gofs + (int64_t)bp * p.np_rt + bt;
             if (CNT) atomicAdd((unsigned long long*)(p.out_cnt + g), (unsigned long long)cnt);
             unsafeAtomicAdd(p.out_w + g, sw);
@@ -614,9 +663,10 @@ constexpr int tile_waves_per_eu() {
     return vgpr_fit && lds_fit ? 8 : 1;
 }
 
-template <int MODE, int PPL, int WAVES, bool CNT, bool POW2, int R, bool MERGE>
+template <int MODE, int PPL, int WAVES, bool CNT, bool POW2, int R, bool MERGE, bool BANDED>
 __global__ void __launch_bounds__(WAVES * 64)
-    __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<MODE, PPL, CNT, R, WAVES>()))) xi_kernel(const lya_xi_problem p, int ntpad) {
+    __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<MODE, PPL, CNT, R, WAVES>()))) xi_kernel(const lya_xi_problem p, int ntpad,
+                                                                                              const BandShape bs) {
     using L = Layout<MODE, CNT>;
     // statically sized, so histogram addresses fold into the ds offset field
     __shared__ __align__(16) unsigned char lds[L::bytes(R)];
@@ -648,33 +698,46 @@ __global__ void __launch_bounds__(WAVES * 64)
         if (tile < 0) break;
         const int ib = cptr(p.tiles)[2 * tile], ie = cptr(p.tiles)[2 * tile + 1];
         const int slot = cptr(p.items)[4 * ib + 3];
-        for (int it = ib; it < ie; ++it) {
-            TaskCtx t;
-            t.f1 = cptr(p.items)[4 * it + 0];
-            const int nb0 = cptr(p.items)[4 * it + 1];
-            const int nb1 = cptr(p.items)[4 * it + 2];
-            t.slot = slot;
-            t.p1 = cptr(p.pix_off)[t.f1];
-            t.n1 = (int)(cptr(p.pix_off)[t.f1 + 1] - t.p1);
-            const int nslab = (t.n1 + 64 * PPL - 1) / (64 * PPL);
-            const int ntask = nslab * (nb1 - nb0);
-            for (int k = wave; k < ntask; k += WAVES) {
-                const int q = k / nslab;
-                const int slab = k - q * nslab;
-                const int f2 = cptr(p.nb_idx)[nb0 + q];
-                run_task<MODE, PPL, CNT, POW2, MERGE>(p, lds, hbase, ntpad4, t, slab, f2, lane, perm, bp64, bt64);
+        // the tile is walked once per band of the binning (once when it fits the LDS)
+        const int nbands = BANDED ? bs.nbp * bs.nbt : 1;
+        for (int band = 0; band < nbands; ++band) {
+            Band bd{0u, 0u, (uint32_t)p.np_rp, (uint32_t)p.np_rt};
+            if constexpr (BANDED) {
+                const int bpi = band / bs.nbt, bti = band - bpi * bs.nbt;
+                bd.bp0 = (uint32_t)(bpi * bs.br);
+                bd.bt0 = (uint32_t)(bti * bs.bc);
+                bd.rows = (uint32_t)min(bs.br, p.np_rp - (int)bd.bp0);
+                bd.cols = (uint32_t)min(bs.bc, p.np_rt - (int)bd.bt0);
             }
-            if constexpr (MODE == LYA_MODE_MIXED_F32) {
-                // every wave done with the item; then the partials move to FP64 and are
-                // zeroed before any wave starts the next item
-                __syncthreads();
-                flush_item(lds, R, nb_lds, tid, NTHR);
-                __syncthreads();
+            for (int it = ib; it < ie; ++it) {
+                TaskCtx t;
+                t.f1 = cptr(p.items)[4 * it + 0];
+                const int nb0 = cptr(p.items)[4 * it + 1];
+                const int nb1 = cptr(p.items)[4 * it + 2];
+                t.slot = slot;
+                t.p1 = cptr(p.pix_off)[t.f1];
+                t.n1 = (int)(cptr(p.pix_off)[t.f1 + 1] - t.p1);
+                const int nslab = (t.n1 + 64 * PPL - 1) / (64 * PPL);
+                const int ntask = nslab * (nb1 - nb0);
+                for (int k = wave; k < ntask; k += WAVES) {
+                    const int q = k / nslab;
+                    const int slab = k - q * nslab;
+                    const int f2 = cptr(p.nb_idx)[nb0 + q];
+                    run_task<MODE, PPL, CNT, POW2, MERGE, BANDED>(p, lds, hbase, ntpad4, t, slab, f2, lane, perm,
+                                                                  bp64, bt64, bd);
+                }
+                if constexpr (MODE == LYA_MODE_MIXED_F32) {
+                    // every wave done with the item; then the partials move to FP64 and are
+                    // zeroed before any wave starts the next item
+                    __syncthreads();
+                    flush_item(lds, R, nb_lds, tid, NTHR);
+                    __syncthreads();
+                }
             }
+            __syncthreads();
+            flush_tile<MODE, CNT>(p, lds, R, ntpad, slot, tid, NTHR, bd);
+            __syncthreads();
         }
-        __syncthreads();
-        flush_tile<MODE, CNT>(p, lds, R, ntpad, slot, tid, NTHR);
-        __syncthreads();
     }
 }
 
@@ -1057,8 +1120,10 @@ __global__ void __launch_bounds__(WAVES * 64) xi_walk_kernel(const lya_xi_proble
 //   At 56-62 VGPRs and (nb + 1) * 16 bytes of LDS it holds 8 waves per SIMD (four
 //   8-wave workgroups per CU at 50 x 50 bins); the compiler keeps the SGPRs within
 //   that occupancy.
+//   GTAB: binnings whose table exceeds a workgroup's LDS read alpha and beta from
+//   global memory (L1/L2-cached), with out-of-range pairs selecting zero.
 // ---------------------------------------------------------------------------
-template <int MODE, int WAVES, int PPL, bool POW2>
+template <int MODE, int WAVES, int PPL, bool POW2, bool GTAB>
 __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(8))) xi_adjoint_kernel(const lya_xi_problem p, const double* alpha,
                                                                 const double* beta, double* g_dw, double* g_w) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -1071,8 +1136,11 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
     const int NP = p.np_rp, NT = p.np_rt;
     const int nb = NP * NT;
     double2* tab = (double2*)lds;   // [nb + 1] (alpha, beta); entry nb is zero
-    for (int b = tid; b <= nb; b += NTHR) tab[b] = b < nb ? make_double2(alpha[b], beta[b]) : make_double2(0.0, 0.0);
-    __syncthreads();
+    if constexpr (!GTAB) {
+        for (int b = tid; b <= nb; b += NTHR)
+            tab[b] = b < nb ? make_double2(alpha[b], beta[b]) : make_double2(0.0, 0.0);
+        __syncthreads();
+    }
     const double bp64 = (double)NP / p.rpmax;
     const double bt64 = (double)NT / p.rtmax;
 
@@ -1119,9 +1187,14 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
                 }
                 // in-range pairs gather alpha, beta of their bin; others read bin 0 times zero
                 auto gather = [&](int m, uint32_t bp, uint32_t bt, double dwj, double wj) {
-                    const uint32_t A = (bp < (uint32_t)NP && bt < (uint32_t)NT) ? __umul24(bp, (uint32_t)NT) + bt
-                                                                              : (uint32_t)nb;
-                    const double2 ab = tab[A];
+                    const bool in = bp < (uint32_t)NP && bt < (uint32_t)NT;
+                    double2 ab;
+                    if constexpr (GTAB) {
+                        const uint32_t A = in ? __umul24(bp, (uint32_t)NT) + bt : 0u;
+                        ab = make_double2(in ? alpha[A] : 0.0, in ? beta[A] : 0.0);
+                    } else {
+                        ab = tab[in ? __umul24(bp, (uint32_t)NT) + bt : (uint32_t)nb];
+                    }
                     gd[m] = fma(ab.x, dwj, gd[m]);
                     gw[m] = fma(ab.y, wj, gw[m]);
                 };
@@ -1276,45 +1349,81 @@ __global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ d
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-using XiKernel = void (*)(const lya_xi_problem, int);
+using XiKernel = void (*)(const lya_xi_problem, int, const BandShape);
 
+// Banded kernels (binnings larger than the LDS histogram) are built for one
+// workgroup shape: 8 waves, one histogram replica, fp64 and mixed modes.
 template <int MODE, int PPL, bool CNT, bool POW2, int R, bool MERGE>
-XiKernel pick_w(int waves) {
-    if constexpr (R == 1) {
-        if (waves == 4) return xi_kernel<MODE, PPL, 4, CNT, POW2, 1, MERGE>;
-        if (waves == 16) return xi_kernel<MODE, PPL, 16, CNT, POW2, 1, MERGE>;
+XiKernel pick_w(int waves, bool banded) {
+    if (banded) {
+        if constexpr (R == 1 && MODE != LYA_MODE_MIXED_F32)
+            if (waves == 8) return xi_kernel<MODE, PPL, 8, CNT, POW2, 1, MERGE, true>;
+        return nullptr;
     }
-    if (waves == 8) return xi_kernel<MODE, PPL, 8, CNT, POW2, R, MERGE>;
+    if constexpr (R == 1) {
+        if (waves == 4) return xi_kernel<MODE, PPL, 4, CNT, POW2, 1, MERGE, false>;
+        if (waves == 16) return xi_kernel<MODE, PPL, 16, CNT, POW2, 1, MERGE, false>;
+    }
+    if (waves == 8) return xi_kernel<MODE, PPL, 8, CNT, POW2, R, MERGE, false>;
     return nullptr;
 }
 template <int MODE, bool CNT, bool POW2, int R, bool MERGE>
-XiKernel pick_p(int ppl, int waves) {
-    if (ppl == 1) return pick_w<MODE, 1, CNT, POW2, R, MERGE>(waves);
-    if (ppl == 2) return pick_w<MODE, 2, CNT, POW2, R, MERGE>(waves);
-    if (ppl == 4) return pick_w<MODE, 4, CNT, POW2, R, MERGE>(waves);
+XiKernel pick_p(int ppl, int waves, bool banded) {
+    if (ppl == 1) return pick_w<MODE, 1, CNT, POW2, R, MERGE>(waves, banded);
+    if (ppl == 2) return pick_w<MODE, 2, CNT, POW2, R, MERGE>(waves, banded);
+    if (ppl == 4) return pick_w<MODE, 4, CNT, POW2, R, MERGE>(waves, banded);
     return nullptr;
 }
 template <int MODE, bool POW2, bool MERGE>
-XiKernel pick_r(bool cnt, int R, int ppl, int waves) {
-    if (R == 1) return cnt ? pick_p<MODE, true, POW2, 1, MERGE>(ppl, waves) : pick_p<MODE, false, POW2, 1, MERGE>(ppl, waves);
-    if (R == 2) return cnt ? pick_p<MODE, true, POW2, 2, MERGE>(ppl, waves) : pick_p<MODE, false, POW2, 2, MERGE>(ppl, waves);
+XiKernel pick_r(bool cnt, int R, int ppl, int waves, bool banded) {
+    if (R == 1)
+        return cnt ? pick_p<MODE, true, POW2, 1, MERGE>(ppl, waves, banded)
+                   : pick_p<MODE, false, POW2, 1, MERGE>(ppl, waves, banded);
+    if (R == 2)
+        return cnt ? pick_p<MODE, true, POW2, 2, MERGE>(ppl, waves, banded)
+                   : pick_p<MODE, false, POW2, 2, MERGE>(ppl, waves, banded);
     return nullptr;
 }
 template <int MODE, bool POW2>
-XiKernel pick_c(bool cnt, int R, int ppl, int waves, bool merge) {
+XiKernel pick_c(bool cnt, int R, int ppl, int waves, bool merge, bool banded) {
     if constexpr (MODE == LYA_MODE_MIXED_F32) {
         (void)merge;
-        return pick_r<MODE, POW2, false>(cnt, R, ppl, waves);
+        return pick_r<MODE, POW2, false>(cnt, R, ppl, waves, banded);
     } else {
-        return merge ? pick_r<MODE, POW2, true>(cnt, R, ppl, waves) : pick_r<MODE, POW2, false>(cnt, R, ppl, waves);
+        return merge ? pick_r<MODE, POW2, true>(cnt, R, ppl, waves, banded)
+                     : pick_r<MODE, POW2, false>(cnt, R, ppl, waves, banded);
     }
 }
 
+using WalkKernel = void (*)(const lya_xi_problem, int);
 template <bool CNT, bool POW2>
-XiKernel pick_walk(int waves) {
+WalkKernel pick_walk(int waves) {
     if (waves == 8) return xi_walk_kernel<8, CNT, POW2>;
     if (waves == 16) return xi_walk_kernel<16, CNT, POW2>;
     return nullptr;
+}
+
+// Band grid for a histogram of rows x (columns + pad) LDS entries within HS: whole
+// rows in column bands while a row of one column fits, else blocks of at most 48
+// columns; bands of equal size.  Returns false when the binning fits unbanded.
+bool band_shape(int np_rp, int np_rt, int pad, BandShape& bs) {
+    bs = BandShape{np_rp, np_rt, 1, 1};
+    if ((int64_t)np_rp * (np_rt + pad) <= HS) return false;
+    if ((int64_t)np_rp * (1 + pad) <= HS) {
+        const int bcmax = HS / np_rp - pad;
+        bs.nbt = (np_rt + bcmax - 1) / bcmax;
+        bs.bc = (np_rt + bs.nbt - 1) / bs.nbt;
+        bs.br = np_rp;
+        bs.nbp = 1;
+    } else {
+        bs.bc = std::min(np_rt, 48);
+        bs.nbt = (np_rt + bs.bc - 1) / bs.bc;
+        bs.bc = (np_rt + bs.nbt - 1) / bs.nbt;
+        const int brmax = HS / (bs.bc + pad);
+        bs.nbp = (np_rp + brmax - 1) / brmax;
+        bs.br = (np_rp + bs.nbp - 1) / bs.nbp;
+    }
+    return true;
 }
 
 bool is_pow2(double x) {
@@ -1324,7 +1433,10 @@ bool is_pow2(double x) {
 
 struct Resolved {
     XiKernel fn;
+    WalkKernel walk_fn;
     int mode, ppl, waves, R, ntpad, groups;
+    bool banded;
+    BandShape bs;
     uint32_t lds;
 };
 
@@ -1342,6 +1454,23 @@ int resolve(const lya_xi_problem* p, const lya_xi_config* cfg, Resolved& r) {
     const bool pow2 = p && is_pow2((double)p->np_rp / p->rpmax) && is_pow2((double)p->np_rt / p->rtmax);
     const bool walk = cfg && cfg->walk;
     const bool merge = cfg && cfg->merge;
+    // LDS columns past the last bin column of a row: merged runs park out-of-range
+    // runs in the padding column, so they need at least one
+    const int padc = walk ? 1 : (merge && r.mode != LYA_MODE_MIXED_F32 ? std::max(pad, 1) : pad);
+    r.banded = false;
+    r.bs = BandShape{0, 0, 1, 1};
+    r.fn = nullptr;
+    r.walk_fn = nullptr;
+    if (p) {
+        if (p->np_rp < 1 || p->np_rt < 1 || !(p->rpmax > 0) || !(p->rtmax > 0))
+            return lya_fail(LYA_ERR_ARG, "bad binning");
+        if (!walk) r.banded = band_shape(p->np_rp, p->np_rt, padc, r.bs);
+        if (r.banded && (r.mode == LYA_MODE_MIXED_F32 || r.R != 1 || r.waves != 8))
+            return lya_fail(LYA_ERR_UNSUPPORTED,
+                            "np_rp*(np_rt+pad)=%lld exceeds the LDS histogram (%d entries): binnings that large "
+                            "run in bands, with the fp64 or mixed mode, 8 waves and one replica",
+                            (long long)p->np_rp * (p->np_rt + padc), HS);
+    }
     if (walk) {
         if (r.mode != LYA_MODE_FP64) return lya_fail(LYA_ERR_UNSUPPORTED, "walk 1 runs the FP64 mode only");
         if (p && p->max_len > WALK_MAX_LEN)
@@ -1349,27 +1478,25 @@ int resolve(const lya_xi_problem* p, const lya_xi_config* cfg, Resolved& r) {
         if (p && (p->pad < p->max_len || p->max_len < 0))
             return lya_fail(LYA_ERR_ARG, "walk 1 needs pad (%d) >= max_len (%d)", p->pad, p->max_len);
         r.R = 1;
-        r.fn = pow2 ? (cnt ? pick_walk<true, true>(r.waves) : pick_walk<false, true>(r.waves))
-                    : (cnt ? pick_walk<true, false>(r.waves) : pick_walk<false, false>(r.waves));
+        r.walk_fn = pow2 ? (cnt ? pick_walk<true, true>(r.waves) : pick_walk<false, true>(r.waves))
+                         : (cnt ? pick_walk<true, false>(r.waves) : pick_walk<false, false>(r.waves));
     } else if (r.mode == LYA_MODE_FP64)
-        r.fn = pow2 ? pick_c<LYA_MODE_FP64, true>(cnt, r.R, r.ppl, r.waves, merge)
-                    : pick_c<LYA_MODE_FP64, false>(cnt, r.R, r.ppl, r.waves, merge);
+        r.fn = pow2 ? pick_c<LYA_MODE_FP64, true>(cnt, r.R, r.ppl, r.waves, merge, r.banded)
+                    : pick_c<LYA_MODE_FP64, false>(cnt, r.R, r.ppl, r.waves, merge, r.banded);
     else if (r.mode == LYA_MODE_MIXED)
-        r.fn = pick_c<LYA_MODE_MIXED, false>(cnt, r.R, r.ppl, r.waves, merge);
+        r.fn = pick_c<LYA_MODE_MIXED, false>(cnt, r.R, r.ppl, r.waves, merge, r.banded);
     else
-        r.fn = pick_c<LYA_MODE_MIXED_F32, false>(cnt, r.R, r.ppl, r.waves, merge);
-    if (!r.fn) return lya_fail(LYA_ERR_ARG, "unsupported shape px_per_lane=%d waves=%d", r.ppl, r.waves);
+        r.fn = pick_c<LYA_MODE_MIXED_F32, false>(cnt, r.R, r.ppl, r.waves, merge, r.banded);
+    if (!r.fn && !r.walk_fn) return lya_fail(LYA_ERR_ARG, "unsupported shape px_per_lane=%d waves=%d", r.ppl, r.waves);
     if (p) {
-        if (p->np_rp < 1 || p->np_rt < 1 || !(p->rpmax > 0) || !(p->rtmax > 0))
-            return lya_fail(LYA_ERR_ARG, "bad binning");
-        // merged runs park out-of-range runs in the padding column: at least one
-        r.ntpad = p->np_rt + (walk ? 1 : (merge && r.mode != LYA_MODE_MIXED_F32 ? std::max(pad, 1) : pad));
+        // LDS row stride: the band's columns (all of them when unbanded) plus padding
+        r.ntpad = (r.banded ? r.bs.bc : p->np_rt) + padc;
         if (walk && (int64_t)(p->np_rp + 1) * r.ntpad > WALK_HS)
-            return lya_fail(LYA_ERR_UNSUPPORTED, "walk 1: (np_rp+1)*(np_rt+1)=%d exceeds %d LDS bins",
+            return lya_fail(LYA_ERR_UNSUPPORTED, "walk 1: (np_rp+1)*(np_rt+1)=%d exceeds %d LDS bins (use walk 0)",
                             (p->np_rp + 1) * r.ntpad, WALK_HS);
-        if ((int64_t)p->np_rp * r.ntpad > HS)
-            return lya_fail(LYA_ERR_UNSUPPORTED, "np_rp*(np_rt+pad)=%d exceeds LDS histogram capacity %d",
-                            p->np_rp * r.ntpad, HS);
+        if ((int64_t)r.bs.br * r.ntpad > HS)
+            return lya_fail(LYA_ERR_UNSUPPORTED, "band of %d x %d bins exceeds the LDS histogram (%d entries)",
+                            r.bs.br, r.ntpad, HS);
     } else {
         r.ntpad = 0;
     }
@@ -1384,7 +1511,8 @@ int resolve(const lya_xi_problem* p, const lya_xi_config* cfg, Resolved& r) {
     if (r.lds > prop.sharedMemPerBlock)
         return lya_fail(LYA_ERR_UNSUPPORTED, "LDS %u bytes > %zu per workgroup", r.lds, prop.sharedMemPerBlock);
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)r.fn, r.waves * 64, 0);
+    const void* kfn = r.walk_fn ? (const void*)r.walk_fn : (const void*)r.fn;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, r.waves * 64, 0);
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     r.groups = (cfg && cfg->groups > 0) ? cfg->groups : per_cu * prop.multiProcessorCount;
     return LYA_OK;
@@ -1413,7 +1541,10 @@ int launch_all(const lya_xi_problem* p, const Resolved& r, hipStream_t st) {
     hipLaunchKernelGGL(zero_outputs, dim3(zb), dim3(256), 0, st, p->out_cnt, p->out_w, p->out_dw, n, p->queue);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return lya_fail(LYA_ERR_HIP, "zero_outputs launch: %s", hipGetErrorString(e));
-    hipLaunchKernelGGL(r.fn, dim3(r.groups), dim3(r.waves * 64), 0, st, *p, r.ntpad);
+    if (r.walk_fn)
+        hipLaunchKernelGGL(r.walk_fn, dim3(r.groups), dim3(r.waves * 64), 0, st, *p, r.ntpad);
+    else
+        hipLaunchKernelGGL(r.fn, dim3(r.groups), dim3(r.waves * 64), 0, st, *p, r.ntpad, r.bs);
     e = hipGetLastError();
     if (e != hipSuccess) return lya_fail(LYA_ERR_HIP, "xi kernel launch: %s", hipGetErrorString(e));
     return LYA_OK;
@@ -1534,20 +1665,28 @@ extern "C" int lya_xi_adjoint(const lya_xi_problem* p, const lya_xi_config* cfg,
     int rc = check_problem(p, mode);
     if (rc) return rc;
     if (!alpha || !beta || !g_dw || !g_w) return lya_fail(LYA_ERR_ARG, "null adjoint buffer");
-    const int nb = p->np_rp * p->np_rt;
-    const size_t lds = (size_t)(nb + 1) * 16;
+    if (p->np_rp < 1 || p->np_rt < 1 || !(p->rpmax > 0) || !(p->rtmax > 0))
+        return lya_fail(LYA_ERR_ARG, "bad binning");
+    const int64_t nb = (int64_t)p->np_rp * p->np_rt;
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
         return lya_fail(LYA_ERR_HIP, "no HIP device");
-    if (lds > prop.sharedMemPerBlock) return lya_fail(LYA_ERR_UNSUPPORTED, "too many bins for the adjoint");
+    // the (alpha, beta) table in LDS when it fits a workgroup, else read from global memory
+    const bool gtab = (size_t)(nb + 1) * 16 > prop.sharedMemPerBlock;
+    const size_t lds = gtab ? 0 : (size_t)(nb + 1) * 16;
     // one forest-1 pixel per lane: two (128-pixel tasks) need ~160 VGPRs for the batched
     // bin gathers, 3 waves per SIMD, and measured 38 % slower
     const bool pow2 = is_pow2((double)p->np_rp / p->rpmax) && is_pow2((double)p->np_rt / p->rtmax);
-    auto fn = (mode == LYA_MODE_FP64) ? (pow2 ? xi_adjoint_kernel<LYA_MODE_FP64, 8, 1, true>
-                                              : xi_adjoint_kernel<LYA_MODE_FP64, 8, 1, false>)
-                                      : xi_adjoint_kernel<LYA_MODE_MIXED, 8, 1, false>;
-    hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    auto fn = (mode == LYA_MODE_FP64)
+                  ? (pow2 ? (gtab ? xi_adjoint_kernel<LYA_MODE_FP64, 8, 1, true, true>
+                                  : xi_adjoint_kernel<LYA_MODE_FP64, 8, 1, true, false>)
+                          : (gtab ? xi_adjoint_kernel<LYA_MODE_FP64, 8, 1, false, true>
+                                  : xi_adjoint_kernel<LYA_MODE_FP64, 8, 1, false, false>))
+                  : (gtab ? xi_adjoint_kernel<LYA_MODE_MIXED, 8, 1, false, true>
+                          : xi_adjoint_kernel<LYA_MODE_MIXED, 8, 1, false, false>);
+    if (hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return lya_fail(LYA_ERR_HIP, "adjoint: cannot reserve %zu bytes of LDS", lds);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, 512, lds) != hipSuccess || per_cu < 1)
         per_cu = 1;

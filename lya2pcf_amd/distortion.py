@@ -70,7 +70,8 @@ class DistortionEngine:
         nbytes = lib().lya_distortion_scratch_bytes(self.groups, self.max_len, self.nb)
         if nbytes < 0:
             raise _lib.LyaError("bad distortion sizes")
-        self.scratch = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
+        # zeroed once: words 1 and 2 count skipped pairs over every run
+        self.scratch = torch.zeros(int(nbytes), dtype=torch.uint8, device=dev)
         self.D = torch.zeros((self.nb, self.nb), dtype=torch.float64, device=dev)
         self.wA = torch.zeros(self.nb, dtype=torch.float64, device=dev)
 
@@ -92,10 +93,22 @@ class DistortionEngine:
                                        self.scratch.numel(), self.groups, _lib.stream_ptr(stream)))
         self._keep = pt
 
+    def skipped(self):
+        """(pairs with a forest longer than max_len, pairs with more active bins than the
+        LDS holds) skipped since the engine was made; both are zero unless a limit was hit."""
+        w = self.scratch[:16].cpu().numpy().view(np.uint32)
+        return int(w[_lib.LYA_DIST_SKIP_LONG]), int(w[_lib.LYA_DIST_SKIP_BINS])
+
     def result(self):
-        """(D, wA) on the host; D is completed from its accumulated upper triangle."""
+        """(D, wA) on the host; D is completed from its accumulated upper triangle.
+        Raises LyaError when a pair was skipped (the sums would be incomplete)."""
         check(lib().lya_distortion_mirror(ptr(self.D), self.nb, _lib.stream_ptr(None)))
         self.torch.cuda.synchronize()
+        long_, bins = self.skipped()
+        if long_ or bins:
+            raise _lib.LyaError("distortion: %d forest pairs skipped (%d with a forest longer than %d pixels, "
+                                "%d with more active bins than the LDS holds at %d bins)"
+                                % (long_ + bins, long_, self.max_len, bins, self.nb), _lib.LYA_ERR_UNSUPPORTED)
         return self.D.cpu().numpy(), self.wA.cpu().numpy()
 
 

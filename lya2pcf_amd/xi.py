@@ -17,7 +17,7 @@ import warnings
 import numpy as np
 
 from . import _lib
-from ._lib import XiProblem, XiConfig, LYA_MODE_FP64, LYA_MODE_MIXED, LYA_MODE_MIXED_F32, check, lib, ptr
+from ._lib import XiProblem, XiConfig, LYA_MODE_FP64, LYA_MODE_MIXED, LYA_MODE_MIXED_F32, LYA_MAX_BINS, check, lib, ptr
 
 MODES = {"fp64": LYA_MODE_FP64, "mixed": LYA_MODE_MIXED, "mixed32": LYA_MODE_MIXED_F32}
 # dc of the padding around the pixel arrays: any pair with it falls outside the bins

@@ -15,14 +15,18 @@ from lya2pcf_amd import forests, synth
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _workspace(tmp_path, n=150, seed=7, **kw):
-    """tmp dir with the drivers, a parameters.py pointing at tmp paths, and data1.npy."""
+def _workspace(tmp_path, n=150, seed=7, params=(), **kw):
+    """tmp dir with the drivers, a parameters.py pointing at tmp paths (with the lines
+    ``params`` replaced: (old, new) pairs), and data1.npy."""
     for f in ("2pla.py", "distortion.py", "correlation_procedures_hip.py", "distortion_procedures_hip.py",
               "forest_class.py"):
         shutil.copy(os.path.join(ROOT, f), tmp_path / f)
     par = open(os.path.join(ROOT, "parameters.py")).read()
     par = par.replace("data_dir = './deltas_lya2pcf/'", "data_dir = '%s/deltas/'" % tmp_path)
     par = par.replace("corr_dir = './outputs/test/'", "corr_dir = '%s/out/'" % tmp_path)
+    for a, b in params:
+        assert a in par, a
+        par = par.replace(a, b)
     (tmp_path / "parameters.py").write_text(par)
     (tmp_path / "deltas").mkdir()
     s = synth.generate(synth.scaled(synth.CONFIGS["patch2k"], n, seed=seed, **kw))   # nside 32, as parameters.py
@@ -96,6 +100,110 @@ def test_2pla_gpu_outputs(tmp_path, precision):
         np.testing.assert_allclose(w_hist, w0[k], rtol=1e-12)
         np.testing.assert_allclose(dw_hist, d0[k], rtol=1e-9, atol=1e-9 * np.abs(w0[k]).max())
     assert (out / "thread_0_of_1.log").exists()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bin_size,rtmax", [(2, 200), (2, 120)])
+def test_2pla_gpu_large_binnings(tmp_path, bin_size, rtmax):
+    """A parameters.py with 2 Mpc/h bins (100 x 100, and 100 x 60 with rtmax = 120):
+    more bins than the LDS histogram holds; 2pla.py --gpu runs them in bands.  Counts
+    bit-exact, sums as the default binning."""
+    s = _workspace(tmp_path, params=(("bin_size_r = 4", "bin_size_r = %d" % bin_size),
+                                     ("rtmax = rmax", "rtmax = %d" % rtmax)))
+    np_rp, np_rt = 200 // bin_size, rtmax // bin_size
+    for precision in ("fp64", "mixed"):
+        _run(tmp_path, "2pla.py", "--gpu", "--counts", "--precision", precision)
+        out = tmp_path / "out"
+        fs = forests.ForestSet.from_survey(s)
+        am = forests.angmax_from(fs, float(rtmax))
+        off, idx = oracle.pairs(fs, am)
+        c0, w0, d0 = oracle.xi_gpu_form(fs, off, idx, np_rp, np_rt, 200.0, float(rtmax))
+        assert c0.sum() > 0
+        for k, pix in enumerate(fs.slots):
+            w_hist, dw_hist = np.load(out / ("2d_histogram_pixel_%d.npy" % pix))
+            assert w_hist.shape == (np_rp, np_rt)
+            assert np.array_equal(np.load(out / ("2d_paircounts_pixel_%d.npy" % pix)), c0[k]), (precision, pix)
+            np.testing.assert_allclose(w_hist, w0[k], rtol=1e-12)
+            np.testing.assert_allclose(dw_hist, d0[k], rtol=1e-9, atol=1e-9 * np.abs(w0[k]).max())
+
+
+@pytest.mark.gpu
+def test_two_point_per_pixel_any_pixel(tmp_path):
+    """A pixel outside init's compute_pixels is computed when asked for (no silent
+    zeros); a pixel not in the data raises KeyError, as the reference's data[pixel]."""
+    import correlation_procedures_hip as cp
+    s = _workspace(tmp_path, n=120, seed=9)
+    data = np.load(tmp_path / "deltas" / "data1.npy", allow_pickle=True).item()
+    pixels = list(data.keys())
+    fs = forests.ForestSet.from_survey(s)
+    am = forests.angmax_from(fs, 200.0)
+    off, idx = oracle.pairs(fs, am)
+    c0, w0, d0 = oracle.xi_gpu_form(fs, off, idx)
+    cp.init(data, None, (50, 50), am, compute_pixels=pixels[:1], counts=True)
+    for k in [0, len(pixels) - 1, 1]:
+        w, d = cp.two_point_per_pixel(pixels[k])
+        assert np.array_equal(cp.pair_counts(pixels[k]), c0[k]), k
+        np.testing.assert_allclose(w, w0[k], rtol=1e-12)
+    assert c0[len(pixels) - 1].sum() > 0
+    with pytest.raises(KeyError):
+        cp.two_point_per_pixel(-5)
+
+
+@pytest.mark.gpu
+def test_distortion_per_pixel_api(tmp_path):
+    """The reference's own distortion entry point (distortion.py:101-125): init, then
+    distortion_per_pixel(data[pixel]) for every pixel in dict order, summed in FP64 and
+    divided by the summed weights.  Within 1e-10 of the oracle's η algebra."""
+    import distortion_procedures_hip as dp
+    from lya2pcf_amd import distortion as dist
+    s = _workspace(tmp_path, n=60, seed=8, length_range=(30, 60))
+    data = np.load(tmp_path / "deltas" / "data1.npy", allow_pickle=True).item()
+    fs = forests.ForestSet.from_survey(s)
+    am = forests.angmax_from(fs, 200.0)
+    # a fresh subsample stream, as the reference's random.seed(1) at import
+    import random
+    random.seed(1)
+    dp.init(data, None, (50, 50), am, 0.8)
+    D = np.zeros((2500, 2500))
+    wA = np.zeros(2500)
+    for pixel in data:
+        d, w = dp.distortion_per_pixel(data[pixel])
+        assert d.shape == (2500, 2500) and w.shape == (2500,)
+        D += d
+        wA += w
+    off, idx = oracle.pairs(fs, am)
+    pairs = dist.subsample(fs, off, idx, range(fs.nf), 0.8, rng=random.Random(1))
+    D0, w0 = oracle.distortion(fs, pairs)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        E, G = D0 / w0[:, None], D / wA[:, None]
+    ok = w0 > 0
+    assert np.array_equal(np.isfinite(G).all(1), ok)
+    assert np.max(np.abs(G[ok] - E[ok])) <= 1e-10 * np.abs(E[ok]).max()
+
+
+@pytest.mark.gpu
+def test_distortion_driver_large_binning(tmp_path):
+    """distortion.py with 2 Mpc/h bins: D is 10,000 x 10,000; the LDS is budgeted by
+    each pair's active bins."""
+    import random
+    from lya2pcf_amd import distortion as dist
+    s = _workspace(tmp_path, n=30, seed=10, length_range=(20, 40), params=(("bin_size_r = 4", "bin_size_r = 2"),))
+    _run(tmp_path, "distortion.py", "--excluded", "0.5")
+    D = np.load(tmp_path / "out" / "distortion.npy", mmap_mode="r")
+    assert D.shape == (10000, 10000) and D.dtype == np.float64
+    fs = forests.ForestSet.from_survey(s)
+    am = forests.angmax_from(fs, 200.0)
+    off, idx = oracle.pairs(fs, am)
+    pairs = dist.subsample(fs, off, idx, range(fs.nf), 0.5, rng=random.Random(1))
+    D0, w0 = oracle.distortion(fs, pairs, 100, 100)
+    ok = np.nonzero(w0 > 0)[0]
+    assert len(ok) > 0
+    E = D0[ok] / w0[ok, None]
+    G = np.asarray(D[ok])
+    assert np.isfinite(G).all()
+    assert np.max(np.abs(G - E)) <= 1e-10 * np.abs(E).max()
+    del D0
+    assert not np.isfinite(np.asarray(D[np.nonzero(w0 == 0)[0][:50]])).any()
 
 
 @pytest.mark.gpu

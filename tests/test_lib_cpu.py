@@ -89,28 +89,35 @@ def test_missing_library_fails_loudly(tmp_path):
 
 
 def test_histogram_capacity_refused_before_any_device_call():
-    """Binnings beyond the LDS histograms are refused while the configuration is
-    resolved, before the library touches a device or the problem's buffers."""
+    """Binnings beyond the LDS histogram run in bands with the fp64 and mixed tile
+    kernels (8 waves, one replica); other shapes, mixed32 and the diagonal walk refuse
+    them while the configuration is resolved, before the library touches a device or
+    the problem's buffers."""
     import ctypes as C
     L = _lib.lib()
 
-    def run(np_rp, np_rt, **cfg):
+    def run(np_rp, np_rt, mode="fp64", **cfg):
         p = _lib.XiProblem()  # binning only: every buffer pointer stays NULL
         p.np_rp, p.np_rt, p.rpmax, p.rtmax = np_rp, np_rt, 4.0 * np_rp, 4.0 * np_rt
-        return L.lya_xi_run(C.byref(p), C.byref(xi.default_config("fp64", counts=False, **cfg)), None)
+        return L.lya_xi_run(C.byref(p), C.byref(xi.default_config(mode, counts=False, **cfg)), None)
 
-    # tile kernel: np_rp * (np_rt + 1) entries, at most LYA_MAX_BINS = 50 * 51
+    accepted = (_lib.LYA_ERR_ARG, _lib.LYA_ERR_HIP)   # NULL buffers, or no device here
+    # one histogram: np_rp * (np_rt + 1) entries, at most LYA_MAX_BINS = 50 * 51
     assert _lib.LYA_MAX_BINS == 50 * 51
-    assert run(51, 50) == _lib.LYA_ERR_UNSUPPORTED
-    assert b"2601" in L.lya_last_error()
-    assert run(50, 51) == _lib.LYA_ERR_UNSUPPORTED
-    # the default 50 x 50 binning fits exactly: refused later, for the NULL buffers
-    # (or for want of a device here), never for capacity
-    assert run(50, 50) in (_lib.LYA_ERR_ARG, _lib.LYA_ERR_HIP)
-    assert run(50, 49, row_pad=0) in (_lib.LYA_ERR_ARG, _lib.LYA_ERR_HIP)
+    assert run(50, 50) in accepted
+    # larger binnings: banded tile kernels
+    for np_rp, np_rt in ((51, 50), (100, 100), (100, 37), (200, 200), (1400, 30), (3000, 3)):
+        for mode in ("fp64", "mixed"):
+            for merge in (0, 1):
+                assert run(np_rp, np_rt, mode, merge=merge) in accepted, (np_rp, np_rt, mode, merge)
+    for bad in (dict(mode="mixed32"), dict(waves=4), dict(waves=16), dict(replicas=2)):
+        assert run(100, 100, **bad) == _lib.LYA_ERR_UNSUPPORTED, bad
+        assert b"bands" in L.lya_last_error()
+        assert run(50, 50, **bad) in accepted, bad
     # merged runs always take the padded stride: 50 x 51 bins fit unpadded only
-    assert run(50, 51, row_pad=0) in (_lib.LYA_ERR_ARG, _lib.LYA_ERR_HIP)
-    assert run(50, 51, row_pad=0, merge=1) == _lib.LYA_ERR_UNSUPPORTED
+    assert run(50, 51, row_pad=0) in accepted
+    assert run(50, 51, "mixed32", row_pad=0) in accepted
+    assert run(50, 51, "mixed32", row_pad=1) == _lib.LYA_ERR_UNSUPPORTED
     # diagonal walk: (np_rp + 1) * (np_rt + 1) LDS bins, at most 2688
     assert run(52, 51, walk=1, waves=16) == _lib.LYA_ERR_UNSUPPORTED
 
