@@ -255,6 +255,26 @@ int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, const double* d
                        int32_t max_len, int32_t np_rp, int32_t np_rt, double rpmax, double rtmax,
                        double* D, double* wA, void* scratch, int64_t scratch_bytes, int32_t groups,
                        void* stream);
+/*
+ * lya_distortion_run without a scratch argument and without any allocation.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The scratch is a built-in device buffer of LYA_DIST_NOALLOC_BYTES
+ * per GPU, held until the process ends.  Calls needing more, i.e.
+ * lya_distortion_scratch_bytes(groups, max_len, nb) > LYA_DIST_NOALLOC_BYTES
+ * (above 417 workgroups at 50 x 50 bins, above 104 at 100 x 100), return
+ * LYA_ERR_CAPACITY.  Every call on one GPU shares that buffer: calls that
+ * overlap on one GPU (two streams, two threads) race and give wrong results.
+ * skipped: NULL, or a caller-owned device uint32[2] to which the pairs skipped
+ * (forest longer than max_len, more active bins than the LDS holds) are added.
+ * Otherwise as lya_distortion_run.
+ */
+#define LYA_DIST_NOALLOC_BYTES (32ll << 20)
+int lya_distortion_run_noalloc(int32_t n_forest, const int64_t* pix_off, const double* dc, const double* we,
+                               const double* dl, const double* ux, const double* uy, const double* uz,
+                               const double* omega, const double* odl2, const int32_t* pairs, int32_t n_pairs,
+                               int32_t max_len, int32_t np_rp, int32_t np_rt, double rpmax, double rtmax,
+                               double* D, double* wA, uint32_t* skipped, int32_t groups, void* stream);
 /* D[c][r] = D[r][c] for r < c: completes the symmetric D accumulated by
  * lya_distortion_run (stream-ordered; idempotent). */
 int lya_distortion_mirror(double* D, int32_t nb, void* stream);

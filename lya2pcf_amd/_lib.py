@@ -16,6 +16,7 @@ LYA_MODE_MIXED_F32 = 2
 LYA_MAX_BINS = 2550
 LYA_DIST_SKIP_LONG = 1   # uint32 words of the distortion scratch counting skipped pairs
 LYA_DIST_SKIP_BINS = 2
+LYA_DIST_NOALLOC_BYTES = 32 << 20   # built-in scratch of lya_distortion_run_noalloc
 
 _vp = C.c_void_p
 _i32 = C.c_int32
@@ -88,6 +89,8 @@ def lib():
     L.lya_distortion_scratch_bytes.restype = _i64
     L.lya_distortion_run.argtypes = [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32,
                                      _i32, _i32, _f64, _f64, _vp, _vp, _vp, _i64, _i32, _vp]
+    L.lya_distortion_run_noalloc.argtypes = [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32,
+                                             _i32, _i32, _f64, _f64, _vp, _vp, _vp, _i32, _vp]
     L.lya_distortion_mirror.argtypes = [_vp, _i32, _vp]
     L.lya_segment_sum.argtypes = [_vp, _vp, _i32, _vp, _vp]
     L.lya_debug_halfangles.argtypes = [_vp, _vp, _vp, _i32, _vp]
@@ -111,7 +114,8 @@ def check(rc):
 # Every symbol include/lya2pcf.h declares.
 EXPORTS = ("lya_last_error", "lya_version", "lya_build_pairs", "lya_build_work", "lya_xi_tau", "lya_xi_run",
            "lya_xi_graph_create", "lya_graph_launch", "lya_graph_destroy", "lya_xi_grid", "lya_xi_adjoint",
-           "lya_distortion_scratch_bytes", "lya_distortion_run", "lya_distortion_mirror", "lya_segment_sum", "lya_debug_halfangles", "lya_copy_bytes",
+           "lya_distortion_scratch_bytes", "lya_distortion_run", "lya_distortion_run_noalloc",
+           "lya_distortion_mirror", "lya_segment_sum", "lya_debug_halfangles", "lya_copy_bytes",
            "lya_build_tiles", "lya_pairs_device_scratch_bytes", "lya_build_pairs_device",
            "lya_items_device_scratch_bytes", "lya_build_items_device")
 

@@ -45,6 +45,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 
 #include "../../include/lya2pcf.h"
 #include "errors.h"
@@ -91,7 +92,8 @@ struct DArgs {
     int32_t zcap;        // doubles of the LDS chunk buffer
     int32_t npad_max;    // bins padded to the block edge
     double* tscr;        // [groups][4][npad_max]: rank rows of each workgroup's current pair
-    uint32_t* queue;
+    uint32_t* queue;     // pair counter
+    uint32_t* skip;      // [2] pairs skipped: forest longer than max_len, too many active bins
 };
 
 // LDS bytes before the chunk buffer: bin maps, chunk weights, staged pixels.
@@ -144,7 +146,7 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
         if (n1 > a.max_len || n2 > a.max_len) {
             // longer than the staging arrays: skipped, and counted in queue[1]; the barrier
             // keeps thread 0 from drawing the next pair before every wave has read this one
-            if (tid == 0) atomicAdd(a.queue + LYA_DIST_SKIP_LONG, 1u);
+            if (tid == 0) atomicAdd(a.skip, 1u);
             __syncthreads();
             continue;
         }
@@ -211,7 +213,7 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
         // the LDS chunk buffer must hold two Z rows of the pair's active bins (padded to
         // the block edge); a pair with more active bins is skipped and counted in queue[2]
         if ((a.zcap / (((nact + BS - 1) / BS) * BS)) < 2) {
-            if (tid == 0) atomicAdd(a.queue + LYA_DIST_SKIP_BINS, 1u);
+            if (tid == 0) atomicAdd(a.skip + 1, 1u);
             continue;
         }
         // compact bin of pixel pair (i, j), or -1
@@ -388,9 +390,10 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
     }
 }
 
-// Only the work counter: the skip counters add up over runs until the caller clears them.
-__global__ void zero_queue(uint32_t* q) {
-    if (threadIdx.x == 0) q[0] = 0u;
+// The first n words of q (the pair counter, and for the allocation-free entry point
+// the skip counters it keeps in its own scratch).
+__global__ void zero_words(uint32_t* q, int n) {
+    if ((int)threadIdx.x < n) q[threadIdx.x] = 0u;
 }
 
 // D[c][r] = D[r][c] for r < c (one thread per element of the lower triangle's rows).
@@ -409,25 +412,30 @@ extern "C" int64_t lya_distortion_scratch_bytes(int32_t groups, int32_t max_len,
     return 256 + (int64_t)groups * 4 * npad * 8;   // pair queue, then the rank rows per workgroup
 }
 
-extern "C" int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, const double* dc, const double* we,
-                                  const double* dl, const double* ux, const double* uy, const double* uz,
-                                  const double* omega, const double* odl2, const int32_t* pairs, int32_t n_pairs,
-                                  int32_t max_len, int32_t np_rp, int32_t np_rt, double rpmax, double rtmax,
-                                  double* D, double* wA, void* scratch, int64_t scratch_bytes, int32_t groups,
-                                  void* stream) {
+namespace {
+
+// Checks, LDS sizing and launches shared by lya_distortion_run and
+// lya_distortion_run_noalloc: `work` holds the pair counter (first 256 bytes) and the
+// rank rows (groups x 4 x npad doubles); the first `nzero` words of `work` are cleared
+// before the kernel.
+int distortion_launch(int32_t n_forest, const int64_t* pix_off, const double* dc, const double* we,
+                      const double* dl, const double* ux, const double* uy, const double* uz, const double* omega,
+                      const double* odl2, const int32_t* pairs, int32_t n_pairs, int32_t max_len, int32_t np_rp,
+                      int32_t np_rt, double rpmax, double rtmax, double* D, double* wA, void* work,
+                      int64_t work_bytes, uint32_t* skip, int nzero, int32_t groups, void* stream) {
     if (n_pairs < 0 || n_forest < 0) return lya_fail(LYA_ERR_ARG, "distortion: negative sizes");
-    if (!pix_off || !dc || !we || !dl || !ux || !uy || !uz || !omega || !odl2 || !D || !wA || !scratch ||
+    if (!pix_off || !dc || !we || !dl || !ux || !uy || !uz || !omega || !odl2 || !D || !wA || !work ||
         (n_pairs > 0 && !pairs))
         return lya_fail(LYA_ERR_ARG, "distortion: null pointer");
     if (np_rp < 1 || np_rt < 1 || !(rpmax > 0) || !(rtmax > 0)) return lya_fail(LYA_ERR_ARG, "distortion: bad binning");
+    if ((int64_t)np_rp * np_rt > 16384) return lya_fail(LYA_ERR_UNSUPPORTED, "distortion: at most 16384 bins");
     const int NB = np_rp * np_rt;
-    if (NB > 16384) return lya_fail(LYA_ERR_UNSUPPORTED, "distortion: at most 16384 bins");
     if (max_len < 1) return lya_fail(LYA_ERR_ARG, "distortion: max_len < 1");
     if ((int64_t)max_len > INT32_MAX / 48) return lya_fail(LYA_ERR_UNSUPPORTED, "distortion: max_len too large");
     if (groups < 1) return lya_fail(LYA_ERR_ARG, "distortion: groups < 1");
     const int64_t need = lya_distortion_scratch_bytes(groups, max_len, NB);
-    if (scratch_bytes < need)
-        return lya_fail(LYA_ERR_CAPACITY, "distortion: scratch %lld < %lld bytes", (long long)scratch_bytes,
+    if (work_bytes < need)
+        return lya_fail(LYA_ERR_CAPACITY, "distortion: scratch %lld < %lld bytes", (long long)work_bytes,
                         (long long)need);
     int dev = 0;
     hipDeviceProp_t prop;
@@ -454,14 +462,87 @@ extern "C" int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, cons
     a.np_rp = np_rp; a.np_rt = np_rt; a.rpmax = rpmax; a.rtmax = rtmax; a.D = D; a.wA = wA;
     a.zcap = zc;
     a.npad_max = (int32_t)npad_max;
-    a.queue = (uint32_t*)scratch;
-    a.tscr = (double*)((unsigned char*)scratch + 256);
+    a.queue = (uint32_t*)work;
+    a.skip = skip;
+    a.tscr = (double*)((unsigned char*)work + 256);
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(zero_queue, dim3(1), dim3(64), 0, st, a.queue);
+    hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st, a.queue, nzero);
     if (n_pairs > 0) hipLaunchKernelGGL(distortion_kernel, dim3(groups), dim3(DTHREADS), lds, st, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return lya_fail(LYA_ERR_HIP, "distortion launch: %s", hipGetErrorString(e));
     return LYA_OK;
+}
+
+}  // namespace
+
+extern "C" int lya_distortion_run(int32_t n_forest, const int64_t* pix_off, const double* dc, const double* we,
+                                  const double* dl, const double* ux, const double* uy, const double* uz,
+                                  const double* omega, const double* odl2, const int32_t* pairs, int32_t n_pairs,
+                                  int32_t max_len, int32_t np_rp, int32_t np_rt, double rpmax, double rtmax,
+                                  double* D, double* wA, void* work, int64_t work_bytes, int32_t groups,
+                                  void* stream) {
+    // pair counter cleared per call; the skip counters (words 1 and 2) add up until the
+    // caller clears them
+    return distortion_launch(n_forest, pix_off, dc, we, dl, ux, uy, uz, omega, odl2, pairs, n_pairs, max_len, np_rp,
+                             np_rt, rpmax, rtmax, D, wA, work, work_bytes,
+                             work ? (uint32_t*)work + LYA_DIST_SKIP_LONG : nullptr, 1, groups, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant.  The pair counter and rank rows live in `scratch`, a
+// file-scope device array of LYA_DIST_NOALLOC_BYTES (each GPU the library runs on
+// holds its own copy until the process ends).  Host code never passes the array's
+// name to a kernel or a copy: its device address is looked up once per device with
+// hipGetSymbolAddress and kept in g_scratch_ptr.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[LYA_DIST_NOALLOC_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES];
+
+// Device address of this GPU's `scratch` (looked up on the first call per device).
+int noalloc_scratch(void** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return lya_fail(LYA_ERR_HIP, "no HIP device");
+    if (dev >= MAX_DEVICES) return lya_fail(LYA_ERR_UNSUPPORTED, "device %d: at most %d devices", dev, MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return lya_fail(LYA_ERR_HIP, "scratch symbol address on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return LYA_OK;
+}
+}  // namespace
+
+extern "C" int lya_distortion_run_noalloc(int32_t n_forest, const int64_t* pix_off, const double* dc,
+                                          const double* we, const double* dl, const double* ux, const double* uy,
+                                          const double* uz, const double* omega, const double* odl2,
+                                          const int32_t* pairs, int32_t n_pairs, int32_t max_len, int32_t np_rp,
+                                          int32_t np_rt, double rpmax, double rtmax, double* D, double* wA,
+                                          uint32_t* skipped, int32_t groups, void* stream) {
+    if (groups >= 1 && max_len >= 1 && np_rp >= 1 && np_rt >= 1 && (int64_t)np_rp * np_rt <= 16384) {
+        const int64_t need = lya_distortion_scratch_bytes(groups, max_len, np_rp * np_rt);
+        if (need > LYA_DIST_NOALLOC_BYTES)
+            return lya_fail(LYA_ERR_CAPACITY,
+                            "distortion (no allocation): %d workgroups x %d bins need %lld bytes of scratch, more "
+                            "than the %lld of the built-in buffer (use fewer workgroups or lya_distortion_run)",
+                            groups, np_rp * np_rt, (long long)need, (long long)LYA_DIST_NOALLOC_BYTES);
+    }
+    void* work = nullptr;
+    int rc = noalloc_scratch(&work);
+    if (rc) return rc;
+    // the pair counter, and the skip counters when the caller keeps none, start at zero
+    // on every call
+    uint32_t* skip = skipped ? skipped : (uint32_t*)work + LYA_DIST_SKIP_LONG;
+    return distortion_launch(n_forest, pix_off, dc, we, dl, ux, uy, uz, omega, odl2, pairs, n_pairs, max_len, np_rp,
+                             np_rt, rpmax, rtmax, D, wA, work, LYA_DIST_NOALLOC_BYTES, skip, skipped ? 1 : 3, groups,
+                             stream);
 }
 
 extern "C" int lya_distortion_mirror(double* D, int32_t nb, void* stream) {
