@@ -52,6 +52,42 @@ def time_graph(g, eng, steps, warmup, torch):
     return wall, ev
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_c0(threads):
+    """The restated reference CPU loop on the 2,000-forest 10 x 10 deg workload (the
+    reference's own CPU-runnable case): all of it, on `threads` cores and on one core."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+    from parameters import chiquito
+    from lya2pcf_amd import forests, synth, xi
+    cfg = synth.CONFIGS["patch2k"]
+    fs = forests.ForestSet.from_survey(synth.generate(cfg))
+    am = forests.angmax_from(fs, 200.0)
+    off, idx = xi.build_pairs(fs, am)
+    n = np.diff(fs.pix_off)
+    f1 = np.repeat(np.arange(fs.nf), np.diff(off))
+    P = float((n[f1] * n[idx]).sum())
+    out = {"workload": cfg.name, "forests": fs.nf, "pixel_pairs": P}
+    for key, nt in (("all_cores", threads), ("one_core", 1)):
+        t0 = time.perf_counter()
+        oracle.xi_cpu_form(fs, off, idx, am, chiquito, threads=nt)
+        dt = time.perf_counter() - t0
+        out[key] = {"value": P / dt, "seconds": dt, "cores": nt}
+        log("cpu baseline c0 (%d threads): %.3g pixel-pairs/s, %.1f s" % (nt, P / dt, dt))
+    return out
+
+
 def cpu_baseline(fs, eng, angmax, budget_pairs, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -84,9 +120,10 @@ def cpu_baseline(fs, eng, angmax, budget_pairs, threads):
                             "sample": "%d forest-1 lines (%.3g pixel pairs, %.1f s)" % (m1, P1, dt1)}}
 
 
-def bench_distortion(args):
+def bench_distortion(args, emit=True):
     """Distortion matrix (5% subsample, FP64 accumulation) and one chi^2 gradient on
-    20k forests: wall times of the device work (reported, not the headline)."""
+    20k forests: wall times of the device work (reported, not the headline).  With
+    emit=False the figures are returned instead of printed as a JSON line."""
     import random
     import numpy as np
     import torch
@@ -137,6 +174,10 @@ def bench_distortion(args):
     tgm = float(np.median(tg))
     P2 = ae.fwd.pixel_pairs
     log("chi2 gradient: %.3f s for %.4g pixel pairs (forward + 2 adjoint walks)" % (tgm, P2))
+    if not emit:
+        return {"workload": cfg.name, "forests": fs.nf, "subsampled_pairs": int(len(pairs)), "pixel_pairs": P,
+                "distortion_s": td, "distortion_pixel_pairs_per_s": P / td,
+                "chi2_gradient_s": tgm, "gradient_pixel_pairs": P2, "gradient_mode": "fp64"}
     print(json.dumps({"metric": "distortion matrix wall time (5% subsample, FP64)", "value": td, "unit": "s",
                       "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                       "data": "synthetic", "dtype": "f64",
@@ -161,6 +202,8 @@ def main():
     ap.add_argument("--cpu-pairs", type=float, default=1.5e11, help="CPU baseline sample size in pixel pairs")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-distortion", action="store_true",
+                    help="skip the 20k-forest distortion and gradient figures of the default line")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--counts", action="store_true", help="time with per-bin pair counts on")
     ap.add_argument("--only", choices=["mixed", "fp64", "fp64-walk", "mixed32"], default=None,
@@ -169,6 +212,9 @@ def main():
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--replicas", type=int, default=0)
     ap.add_argument("--row-pad", type=int, default=-1)
+    ap.add_argument("--tail-max", type=int, default=0,
+                    help="longest forest-1 tail walked against forest 2 in lanes (0: tuned/default, -1: whole "
+                         "slabs of 64*px_per_lane pixels only)")
     ap.add_argument("--copy-calibrate", type=int, default=0,
                     help="after timing, run one plain device copy of this many bytes (calibrates the "
                          "profiler's HBM byte counters)")
@@ -229,6 +275,8 @@ def main():
                 shape[k] = a
         if args.row_pad >= 0:
             shape["row_pad"] = args.row_pad
+        if args.tail_max and not shape.get("walk"):
+            shape["tail_max"] = args.tail_max
         return base, shape
 
     # Timed launches compute the reference's outputs (sum w1 w2, sum dw1 dw2); one
@@ -251,9 +299,12 @@ def main():
             torch.cuda.synchronize()
             t_cnt = e0.elapsed_time(e1) / 1e3
             res = eng.results()
-        out[mode] = {"wall": wall, "ev": ev, "res": res, "shape": shape, "with_counts_s": t_cnt}
-        log("%s %s: %.4f s/step (events median %.4f s) -> %.4g pixel-pairs/s"
-            % (mode, shape, wall, float(np.median(ev)), P / wall))
+        work = None if mode == "fp64-walk" else eng.work_stats(xi.default_config(base, **shape))
+        out[mode] = {"wall": wall, "ev": ev, "res": res, "shape": shape, "with_counts_s": t_cnt, "work": work,
+                     "binned": float(res[0].sum())}
+        log("%s %s: %.4f s/step (events median %.4f s) -> %.4g pixel-pairs/s; binned %.4g, evaluated %s"
+            % (mode, shape, wall, float(np.median(ev)), P / wall, out[mode]["binned"],
+               "%.4g" % work["evaluated_pairs"] if work else "n/a"))
 
     if args.copy_calibrate:
         from lya2pcf_amd import _lib
@@ -271,6 +322,8 @@ def main():
     ev_med = float(np.median(m["ev"]))
     achieved = P * FLOP_PER_PAIR / ev_med / 1e12
     peak = FP64_PEAK_TFLOPS if main_mode.startswith("fp64") else FP32_PEAK_TFLOPS
+    B = m["binned"]
+    achieved_binned = B * FLOP_PER_PAIR / ev_med / 1e12
     line = {
         "metric": METRIC, "value": value, "unit": "pixel-pairs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": m["wall"] * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -285,11 +338,22 @@ def main():
                      # workload only; other workloads report none
                      "traffic": autotune.measured_traffic(main_mode) if args.config == "sky100k" and
                      not args.forests else None,
-                     "note": "10 FLOP per pixel pair (algorithmic) / median per-launch hipEvent time; "
-                             "no MFMA or HBM bound applies"},
+                     "pixel_pairs": P, "binned_pairs": B,
+                     "evaluated_pairs": m["work"]["evaluated_pairs"] if m["work"] else None,
+                     "binned_per_pixel_pair": B / P,
+                     "evaluated_per_pixel_pair": m["work"]["evaluated_pairs"] / P if m["work"] else None,
+                     "achieved_binned": achieved_binned, "frac_binned": achieved_binned / peak,
+                     "note": "achieved/frac: 10 FLOP per pixel pair of P (algorithmic, pruned pairs included) / "
+                             "median per-launch hipEvent time; achieved_binned/frac_binned: 10 FLOP per binned "
+                             "pair (sum of the per-bin counts); evaluated_pairs: pixel-pair slots the kernel steps "
+                             "through after pruning (lya_xi_work_stats); no MFMA or HBM bound applies"},
         "kernels": {k: {"value": P / v["wall"], "ms_per_step": v["wall"] * 1e3,
                         "event_median_ms": float(np.median(v["ev"])) * 1e3, "shape": v["shape"],
-                        "with_counts_ms": None if v["with_counts_s"] is None else v["with_counts_s"] * 1e3}
+                        "with_counts_ms": None if v["with_counts_s"] is None else v["with_counts_s"] * 1e3,
+                        "binned_pairs": v["binned"],
+                        "evaluated_pairs": v["work"]["evaluated_pairs"] if v["work"] else None,
+                        "frac_binned": v["binned"] * FLOP_PER_PAIR / float(np.median(v["ev"])) / 1e12 /
+                        (FP64_PEAK_TFLOPS if k.startswith("fp64") else FP32_PEAK_TFLOPS)}
                     for k, v in out.items()},
         "setup": setup,
     }
@@ -327,6 +391,14 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log("cpu baseline: %d threads, ~%.2g pixel pairs" % (threads, args.cpu_pairs))
         line["cpu_baseline"] = cpu_baseline(fs, eng, angmax, args.cpu_pairs, threads)
+        line["cpu_baseline"]["cpu_model"] = cpu_model()
+        line["cpu_baseline"]["os_cpu_count"] = os.cpu_count()
+        line["cpu_baseline"]["c0"] = cpu_baseline_c0(threads)
+    if args.config == "sky100k" and not args.forests and not args.no_distortion and not args.only:
+        del eng
+        torch.cuda.empty_cache()
+        args.forests = 0
+        line["distortion"] = bench_distortion(args, emit=False)
     print(json.dumps(line), flush=True)
 
 

@@ -177,6 +177,11 @@ typedef struct lya_xi_config {
     int32_t merge;            /* walk 0, fp64 and mixed: 1 sums runs of equal bins    */
                               /* within each block of forest-2 pixels before the LDS  */
                               /* atomics (one atomic set per run)                     */
+    int32_t tail_max;         /* walk 0.  Forest 1 is cut into slabs of 64*px_per_lane */
+                              /* pixels, then 64-pixel slabs; a last piece of at most */
+                              /* tail_max (< 64) pixels is walked against forest 2 in */
+                              /* lanes (fewer idle lanes).  0: 48; -1: slabs of       */
+                              /* 64*px_per_lane pixels only, the last one partial      */
 } lya_xi_config;
 
 /* Largest mixed-mode error bound for these data (host): returns tau in bin units. */
@@ -201,6 +206,12 @@ int lya_graph_destroy(lya_graph_t g);
 
 /* Workgroups the persistent grid uses for cfg (0 on error). */
 int lya_xi_grid(const lya_xi_config* cfg, int32_t* groups);
+
+/* Work of the tile kernels for (p, cfg) without binning (no counterpart in the
+ * reference; a measurement helper).  out: device uint64[4] = pixel-pair slots
+ * stepped (lanes x pruned forest-2 range, idle lanes included), the same over
+ * lanes holding a pixel, tasks, tasks with a non-empty range.  One band. */
+int lya_xi_work_stats(const lya_xi_problem* p, const lya_xi_config* cfg, uint64_t* out, void* stream);
 
 /*
  * Adjoint of the pair sums.  For every pair (i in forest f1, j in neighbour

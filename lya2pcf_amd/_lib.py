@@ -44,7 +44,7 @@ class XiConfig(C.Structure):
     the problem's ``lane_map``)."""
     _fields_ = [("mode", _i32), ("px_per_lane", _i32), ("waves", _i32), ("replicas", _i32),
                 ("row_pad", _i32), ("groups", _i32), ("no_counts", _i32), ("walk", _i32),
-                ("merge", _i32)]
+                ("merge", _i32), ("tail_max", _i32)]
     lane_map = 1
 
     def as_dict(self):
@@ -84,6 +84,7 @@ def lib():
     L.lya_graph_launch.argtypes = [_vp, _vp]
     L.lya_graph_destroy.argtypes = [_vp]
     L.lya_xi_grid.argtypes = [C.POINTER(XiConfig), C.POINTER(_i32)]
+    L.lya_xi_work_stats.argtypes = [C.POINTER(XiProblem), C.POINTER(XiConfig), _vp, _vp]
     L.lya_xi_adjoint.argtypes = [C.POINTER(XiProblem), C.POINTER(XiConfig), _vp, _vp, _vp, _vp, _vp]
     L.lya_distortion_scratch_bytes.argtypes = [_i32, _i32, _i32]
     L.lya_distortion_scratch_bytes.restype = _i64
@@ -113,7 +114,8 @@ def check(rc):
 
 # Every symbol include/lya2pcf.h declares.
 EXPORTS = ("lya_last_error", "lya_version", "lya_build_pairs", "lya_build_work", "lya_xi_tau", "lya_xi_run",
-           "lya_xi_graph_create", "lya_graph_launch", "lya_graph_destroy", "lya_xi_grid", "lya_xi_adjoint",
+           "lya_xi_graph_create", "lya_graph_launch", "lya_graph_destroy", "lya_xi_grid", "lya_xi_work_stats",
+           "lya_xi_adjoint",
            "lya_distortion_scratch_bytes", "lya_distortion_run", "lya_distortion_run_noalloc",
            "lya_distortion_mirror", "lya_segment_sum", "lya_debug_halfangles", "lya_copy_bytes",
            "lya_build_tiles", "lya_pairs_device_scratch_bytes", "lya_build_pairs_device",

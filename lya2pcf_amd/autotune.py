@@ -39,7 +39,7 @@ SPACE = {"px_per_lane": (1, 2, 4), "replicas": (1,), "row_pad": (1,), "waves": (
 FULL_SPACE = {"px_per_lane": (1, 2, 4), "replicas": (1, 2), "row_pad": (0, 1), "waves": (4, 8, 16),
               "lane_map": (0, 1, 2), "merge": (0, 1)}
 WALK_SPACE = {"walk": (1,), "waves": (8, 16)}
-SHAPE_KEYS = ("px_per_lane", "waves", "replicas", "row_pad", "lane_map", "walk", "merge")
+SHAPE_KEYS = ("px_per_lane", "waves", "replicas", "row_pad", "lane_map", "walk", "merge", "tail_max")
 
 
 def load_best(mode="mixed", config="sky100k", walk=None):

@@ -138,18 +138,19 @@ struct BandShape {
     int32_t br, bc, nbp, nbt;
 };
 
-// Forest-2 j range that can hold an in-range pair for some forest-1 pixel of the
-// slab.  Needs both forests sorted by dc; the margins (1e-3 bin) dominate the
-// FP64 rounding of the thresholds, so skipped pairs are out of range exactly.
+// Forest-2 j range, within [j0, j1), that can hold an in-range pair for some
+// forest-1 pixel of the slab.  Needs both forests sorted by dc; the margins (1e-3
+// bin) dominate the FP64 rounding of the thresholds, so skipped pairs are out of
+// range exactly.
 // BANDED: pairs that land in the band [bp0, bp0 + rows) x [bt0, bt0 + cols) only.
 // r_perp = (r1 + r2) s grows with j, so the band's columns give a lower bound on j
 // as well; the row bound uses the band's last row (rows from 0 upwards, a superset).
 template <bool BANDED = false>
 __device__ __forceinline__ void prune(const lya_xi_problem& p, int f1, int64_t p1, int i0, int ilast, int f2,
-                                      int64_t p2, int n2, double c, double s, double bp64, double bt64, int lane,
-                                      int& jlo, int& jhi, const Band& bd = Band{0u, 0u, 0u, 0u}) {
-    jlo = 0;
-    jhi = n2;
+                                      int64_t p2, int j0, int j1, double c, double s, double bp64, double bt64,
+                                      int lane, int& jlo, int& jhi, const Band& bd = Band{0u, 0u, 0u, 0u}) {
+    jlo = j0;
+    jhi = j1;
     if (!(cptr(p.sorted)[f1] & cptr(p.sorted)[f2])) return;
     const double r1min = cptr(p.dc)[p1 + i0];
     const double r1max = cptr(p.dc)[p1 + ilast];
@@ -163,27 +164,88 @@ __device__ __forceinline__ void prune(const lya_xi_problem& p, int f1, int64_t p
     const double lot = BANDED && bd.bt0 > 0u ? ((double)bd.bt0 - 1e-3) / (s * bt64) - r1max : -INFINITY;
     if constexpr (BANDED) {
         // the forest pair misses the band altogether: nothing to prune
-        if (n2 == 0 || cptr(p.dc)[p2 + n2 - 1] < lot || cptr(p.dc)[p2] >= hi) {
-            jlo = jhi = 0;
+        if (j1 <= j0 || cptr(p.dc)[p2 + j1 - 1] < lot || cptr(p.dc)[p2 + j0] >= hi) {
+            jlo = jhi = j0;
             return;
         }
     }
     int clo = 0, chi = 0;
-    const int nr = (n2 + 63) & ~63;
-    for (int jj = lane; jj < nr; jj += 64) {
-        const double r2 = (jj < n2) ? p.dc[p2 + jj] : INFINITY;
+    const int jr = j0 + ((j1 - j0 + 63) & ~63);
+    for (int jj = j0 + lane; jj < jr; jj += 64) {
+        const double r2 = (jj < j1) ? p.dc[p2 + jj] : INFINITY;
         if constexpr (BANDED)
             clo += __popcll(__ballot(r2 <= lo || r2 < lot));
         else
             clo += __popcll(__ballot(r2 <= lo));
         chi += __popcll(__ballot(r2 < hi));
     }
-    jlo = uni(clo);
-    jhi = uni(chi > clo ? chi : clo);
+    jlo = j0 + uni(clo);
+    jhi = j0 + uni(chi > clo ? chi : clo);
+}
+
+// Work units of forest 1 (n1 pixels) against each neighbour: m slabs of SL = 64*PPL
+// pixels; with split > 0 the rest goes to nv slabs of 64 pixels and a tail of at most
+// `split` (< 64) pixels, walked wave-uniformly against the neighbour's pixels in lanes
+// (64-pixel slabs of forest 2).  A short tail in lanes would leave most lanes idle over
+// a window of ~2 rmax; forest 2 in the lanes keeps them busy.  split = 0: slabs of SL
+// only, the last one partial.
+struct Units {
+    int m, nv, tail, upq;
+};
+__device__ __forceinline__ Units units_of(int n1, int SL, int split) {
+    Units u;
+    if (split > 0) {
+        u.m = n1 / SL;
+        const int r = n1 - u.m * SL;
+        u.nv = r >> 6;
+        u.tail = r & 63;
+        if (u.tail > split) {  // a longer tail takes one more 64-pixel slab
+            ++u.nv;
+            u.tail = 0;
+        }
+    } else {
+        u.m = (n1 + SL - 1) / SL;
+        u.nv = 0;
+        u.tail = 0;
+    }
+    u.upq = u.m + u.nv + (u.tail > 0 ? 1 : 0);
+    return u;
+}
+
+// Arguments of unit `sub` of the pair (forest 1 = t, neighbour f2 at p2 with n2 pixels):
+// the lanes' forest tl from pixel i0 against the pixels [u0, u1) of forest fu.  A tail
+// unit has forest 2 in the lanes and steps i0 through its 64-pixel slabs.
+struct UnitArgs {
+    TaskCtx tl;
+    int fu, u0, u1, i0;
+    bool tail, wide;
+};
+__device__ __forceinline__ UnitArgs unit_args(const TaskCtx& t, const Units& u, int SL, int sub, int f2, int64_t p2,
+                                              int n2) {
+    UnitArgs a;
+    a.tl = t;
+    a.fu = f2;
+    a.u0 = 0;
+    a.u1 = n2;
+    a.wide = sub < u.m;
+    a.i0 = a.wide ? sub * SL : u.m * SL + (sub - u.m) * 64;
+    a.tail = sub >= u.m + u.nv;
+    if (a.tail) {
+        a.tl.f1 = f2;
+        a.tl.p1 = p2;
+        a.tl.n1 = n2;
+        a.fu = t.f1;
+        a.u0 = t.n1 - u.tail;
+        a.u1 = t.n1;
+        a.i0 = 0;
+    }
+    return a;
 }
 
 // ---------------------------------------------------------------------------
-// One task: a 64*PPL-pixel slab of forest 1 against all pixels of forest 2.
+// One task: a 64*PPL-pixel slab of the lanes' forest (t, from pixel i0) against the
+// pixels [u0, u1) of forest f2, walked wave-uniformly.  The bins and products are
+// symmetric in the two forests, so either forest of a pair can take the lanes.
 //   CNT   also count pixel pairs per bin (one more LDS atomic per pair)
 //   POW2  FP64 mode with power-of-two binners: the binner is folded into the
 //         half-angle factors, fl(|d| (c b)) = fl(|d| c) b exactly when b = 2^k,
@@ -193,23 +255,21 @@ __device__ __forceinline__ void prune(const lya_xi_problem& p, int f1, int64_t p
 // ---------------------------------------------------------------------------
 template <int MODE, int PPL, bool CNT, bool POW2, bool MERGE, bool BANDED>
 __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char* lds, uint32_t hbase,
-                                         uint32_t ntpad4, const TaskCtx& t, int slab, int f2, int lane, int perm,
-                                         double bp64, double bt64, const Band& bd) {
+                                         uint32_t ntpad4, const TaskCtx& t, int i0, int f2, int u0, int u1,
+                                         int lane, int perm, double bp64, double bt64, const Band& bd) {
     static_assert(!(BANDED && MODE == LYA_MODE_MIXED_F32), "mixed32 has no banded form");
     using L = Layout<MODE, CNT>;
     // rows and columns of the LDS histogram
     const int NP = BANDED ? (int)bd.rows : p.np_rp, NT = BANDED ? (int)bd.cols : p.np_rt;
     const int64_t p2 = cptr(p.pix_off)[f2];
-    const int n2 = (int)(cptr(p.pix_off)[f2 + 1] - p2);
     double c, s;
     halfangles(cptr(p.ux)[t.f1], cptr(p.uy)[t.f1], cptr(p.uz)[t.f1], cptr(p.ux)[f2], cptr(p.uy)[f2],
                cptr(p.uz)[f2], c, s);
     c = unid(c);
     s = unid(s);
-    const int i0 = slab * 64 * PPL;
     const int ilast = min(t.n1, i0 + 64 * PPL) - 1;
     int jlo, jhi;
-    prune<BANDED>(p, t.f1, t.p1, i0, ilast, f2, p2, n2, c, s, bp64, bt64, lane, jlo, jhi, bd);
+    prune<BANDED>(p, t.f1, t.p1, i0, ilast, f2, p2, u0, u1, c, s, bp64, bt64, lane, jlo, jhi, bd);
     if (jlo >= jhi) return;
 
     double r1[PPL];
@@ -666,7 +726,7 @@ constexpr int tile_waves_per_eu() {
 template <int MODE, int PPL, int WAVES, bool CNT, bool POW2, int R, bool MERGE, bool BANDED>
 __global__ void __launch_bounds__(WAVES * 64)
     __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<MODE, PPL, CNT, R, WAVES>()))) xi_kernel(const lya_xi_problem p, int ntpad,
-                                                                                              const BandShape bs) {
+                                                                                              const BandShape bs, int split) {
     using L = Layout<MODE, CNT>;
     // statically sized, so histogram addresses fold into the ds offset field
     __shared__ __align__(16) unsigned char lds[L::bytes(R)];
@@ -717,14 +777,27 @@ __global__ void __launch_bounds__(WAVES * 64)
                 t.slot = slot;
                 t.p1 = cptr(p.pix_off)[t.f1];
                 t.n1 = (int)(cptr(p.pix_off)[t.f1 + 1] - t.p1);
-                const int nslab = (t.n1 + 64 * PPL - 1) / (64 * PPL);
-                const int ntask = nslab * (nb1 - nb0);
+                constexpr int SL = 64 * PPL;
+                const Units un = units_of(t.n1, SL, split);
+                const int ntask = un.upq * (nb1 - nb0);
                 for (int k = wave; k < ntask; k += WAVES) {
-                    const int q = k / nslab;
-                    const int slab = k - q * nslab;
+                    const int q = k / un.upq;
+                    const int sub = k - q * un.upq;
                     const int f2 = cptr(p.nb_idx)[nb0 + q];
-                    run_task<MODE, PPL, CNT, POW2, MERGE, BANDED>(p, lds, hbase, ntpad4, t, slab, f2, lane, perm,
-                                                                  bp64, bt64, bd);
+                    const int64_t p2 = cptr(p.pix_off)[f2];
+                    const int n2 = (int)(cptr(p.pix_off)[f2 + 1] - p2);
+                    UnitArgs a = unit_args(t, un, SL, sub, f2, p2, n2);
+                    // one call site per slab width (the inlined task body is large)
+                    for (; a.i0 < a.tl.n1; a.i0 += 64) {
+                        if (PPL > 1 && a.wide) {
+                            run_task<MODE, PPL, CNT, POW2, MERGE, BANDED>(p, lds, hbase, ntpad4, a.tl, a.i0, a.fu,
+                                                                          a.u0, a.u1, lane, perm, bp64, bt64, bd);
+                        } else {
+                            run_task<MODE, 1, CNT, POW2, MERGE, BANDED>(p, lds, hbase, ntpad4, a.tl, a.i0, a.fu,
+                                                                        a.u0, a.u1, lane, perm, bp64, bt64, bd);
+                        }
+                        if (!a.tail) break;
+                    }
                 }
                 if constexpr (MODE == LYA_MODE_MIXED_F32) {
                     // every wave done with the item; then the partials move to FP64 and are
@@ -1172,7 +1245,7 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
                 s = unid(s);
                 const int ilast = min(n1, slab * SL + SL) - 1;
                 int jlo, jhi;
-                prune(p, f1, p1, slab * SL, ilast, f2, p2, n2, c, s, bp64, bt64, lane, jlo, jhi);
+                prune(p, f1, p1, slab * SL, ilast, f2, p2, 0, n2, c, s, bp64, bt64, lane, jlo, jhi);
                 if (jlo >= jhi) continue;
                 int i[PPL];
                 bool valid[PPL];
@@ -1309,6 +1382,57 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
     }
 }
 
+// Work the tile kernels do, per the decomposition and pruning above (one band): one
+// wave per item.  out[0] lane slots stepped (lanes x pruned j range, idle lanes
+// included), out[1] the same over lanes holding a pixel, out[2] tasks, out[3] tasks
+// with a non-empty j range.  Each lane slot is one pixel pair evaluated.
+template <int PPL>
+__global__ void work_stats_kernel(const lya_xi_problem p, int split, unsigned long long* out) {
+    constexpr int SL = 64 * PPL;
+    const int lane = threadIdx.x & 63;
+    const int nw = (int)(gridDim.x * blockDim.x) >> 6;
+    const double bp64 = (double)p.np_rp / p.rpmax, bt64 = (double)p.np_rt / p.rtmax;
+    unsigned long long slots = 0, valid = 0, tasks = 0, busy = 0;
+    for (int it = uni((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6)); it < p.n_items; it += nw) {
+        TaskCtx t;
+        t.f1 = p.items[4 * it];
+        const int nb0 = p.items[4 * it + 1], nb1 = p.items[4 * it + 2];
+        t.p1 = p.pix_off[t.f1];
+        t.n1 = (int)(p.pix_off[t.f1 + 1] - t.p1);
+        t.slot = p.items[4 * it + 3];
+        const Units un = units_of(t.n1, SL, split);
+        for (int q = nb0; q < nb1; ++q) {
+            const int f2 = p.nb_idx[q];
+            const int64_t p2 = p.pix_off[f2];
+            const int n2 = (int)(p.pix_off[f2 + 1] - p2);
+            double c, s;
+            halfangles(p.ux[t.f1], p.uy[t.f1], p.uz[t.f1], p.ux[f2], p.uy[f2], p.uz[f2], c, s);
+            for (int sub = 0; sub < un.upq; ++sub) {
+                UnitArgs a = unit_args(t, un, SL, sub, f2, p2, n2);
+                for (; a.i0 < a.tl.n1; a.i0 += 64) {
+                    const int w = (PPL > 1 && a.wide) ? SL : 64;
+                    const int ilast = min(a.tl.n1, a.i0 + w) - 1;
+                    int jlo, jhi;
+                    prune(p, a.tl.f1, a.tl.p1, a.i0, ilast, a.fu, p.pix_off[a.fu], a.u0, a.u1, c, s, bp64, bt64,
+                          lane, jlo, jhi);
+                    const unsigned long long span = (unsigned long long)max(0, jhi - jlo);
+                    slots += (unsigned long long)w * span;
+                    valid += (unsigned long long)(ilast - a.i0 + 1) * span;
+                    tasks += 1;
+                    busy += span > 0;
+                    if (!a.tail) break;
+                }
+            }
+        }
+    }
+    if (lane == 0) {
+        atomicAdd(out, slots);
+        atomicAdd(out + 1, valid);
+        atomicAdd(out + 2, tasks);
+        atomicAdd(out + 3, busy);
+    }
+}
+
 // out[f] = sum of x[off[f] .. off[f+1]), one wave per segment, fixed summation order.
 __global__ void segment_sum_kernel(const double* __restrict__ x, const int64_t* __restrict__ off, int32_t nseg,
                                    double* __restrict__ out) {
@@ -1349,7 +1473,7 @@ __global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ d
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-using XiKernel = void (*)(const lya_xi_problem, int, const BandShape);
+using XiKernel = void (*)(const lya_xi_problem, int, const BandShape, int);
 
 // Banded kernels (binnings larger than the LDS histogram) are built for one
 // workgroup shape: 8 waves, one histogram replica, fp64 and mixed modes.
@@ -1437,6 +1561,7 @@ struct Resolved {
     int mode, ppl, waves, R, ntpad, groups;
     bool banded;
     BandShape bs;
+    int split;
     uint32_t lds;
 };
 
@@ -1454,6 +1579,8 @@ int resolve(const lya_xi_problem* p, const lya_xi_config* cfg, Resolved& r) {
     const bool pow2 = p && is_pow2((double)p->np_rp / p->rpmax) && is_pow2((double)p->np_rt / p->rtmax);
     const bool walk = cfg && cfg->walk;
     const bool merge = cfg && cfg->merge;
+    // longest forest-1 tail walked against forest 2 in lanes (0: whole slabs only)
+    r.split = !cfg || cfg->tail_max == 0 ? 48 : (cfg->tail_max < 0 ? 0 : std::min(cfg->tail_max, 63));
     // LDS columns past the last bin column of a row: merged runs park out-of-range
     // runs in the padding column, so they need at least one
     const int padc = walk ? 1 : (merge && r.mode != LYA_MODE_MIXED_F32 ? std::max(pad, 1) : pad);
@@ -1544,7 +1671,7 @@ int launch_all(const lya_xi_problem* p, const Resolved& r, hipStream_t st) {
     if (r.walk_fn)
         hipLaunchKernelGGL(r.walk_fn, dim3(r.groups), dim3(r.waves * 64), 0, st, *p, r.ntpad);
     else
-        hipLaunchKernelGGL(r.fn, dim3(r.groups), dim3(r.waves * 64), 0, st, *p, r.ntpad, r.bs);
+        hipLaunchKernelGGL(r.fn, dim3(r.groups), dim3(r.waves * 64), 0, st, *p, r.ntpad, r.bs, r.split);
     e = hipGetLastError();
     if (e != hipSuccess) return lya_fail(LYA_ERR_HIP, "xi kernel launch: %s", hipGetErrorString(e));
     return LYA_OK;
@@ -1657,6 +1784,22 @@ extern "C" int lya_segment_sum(const double* x, const int64_t* off, int32_t nseg
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return lya_fail(LYA_ERR_HIP, "segment sum launch: %s", hipGetErrorString(e));
     return LYA_OK;
+}
+
+extern "C" int lya_xi_work_stats(const lya_xi_problem* p, const lya_xi_config* cfg, uint64_t* out, void* stream) {
+    Resolved r;
+    int rc = resolve(p, cfg, r);
+    if (rc) return rc;
+    if ((rc = check_problem(p, r.mode))) return rc;
+    if (!out) return lya_fail(LYA_ERR_ARG, "work stats: null output");
+    if (r.walk_fn) return lya_fail(LYA_ERR_UNSUPPORTED, "work stats: tile kernels only (walk 0)");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(zero_outputs, dim3(1), dim3(64), 0, st, out, (double*)nullptr, (double*)nullptr, (int64_t)4,
+                       p->queue);
+    auto fn = r.ppl == 1 ? work_stats_kernel<1> : (r.ppl == 2 ? work_stats_kernel<2> : work_stats_kernel<4>);
+    hipLaunchKernelGGL(fn, dim3(2048), dim3(256), 0, st, *p, r.split, (unsigned long long*)out);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? LYA_OK : lya_fail(LYA_ERR_HIP, "work stats launch: %s", hipGetErrorString(e));
 }
 
 extern "C" int lya_xi_adjoint(const lya_xi_problem* p, const lya_xi_config* cfg, const double* alpha,

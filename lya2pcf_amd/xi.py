@@ -182,6 +182,7 @@ def default_config(mode="mixed", **kw):
     cfg.lane_map = kw.get("lane_map", 1)
     cfg.walk = kw.get("walk", 0)
     cfg.merge = kw.get("merge", 0)
+    cfg.tail_max = kw.get("tail_max", 0)
     return cfg
 
 
@@ -387,6 +388,18 @@ class XiEngine:
         cfg = cfg or default_config()
         p = self.problem(cfg)
         check(lib().lya_xi_run(C.byref(p), C.byref(cfg), _lib.stream_ptr(stream)))
+
+    def work_stats(self, cfg=None):
+        """Work of the tile kernels for cfg (one band): {'evaluated_pairs': lane slots
+        stepped (pixel pairs evaluated, idle lanes included), 'evaluated_valid': the same
+        over lanes holding a pixel, 'tasks', 'tasks_nonempty'}."""
+        cfg = cfg or default_config()
+        p = self.problem(cfg)
+        out = self.torch.zeros(4, dtype=self.torch.int64, device=self.device)
+        check(lib().lya_xi_work_stats(C.byref(p), C.byref(cfg), ptr(out), _lib.stream_ptr(None)))
+        v = out.cpu().numpy().view(np.uint64)
+        return {"evaluated_pairs": float(v[0]), "evaluated_valid": float(v[1]), "tasks": int(v[2]),
+                "tasks_nonempty": int(v[3])}
 
     def graph(self, cfg=None):
         cfg = cfg or default_config()

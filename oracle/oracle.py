@@ -37,6 +37,7 @@ def lib():
         L.oracle_pairs.restype = _i64
         L.oracle_pairs.argtypes = [_i32, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _vp]
         L.oracle_xi_gpu_form.argtypes = [_vp] * 11 + [_i32, _i32, _i32, _f64, _f64, _vp, _vp, _vp]
+        L.oracle_xi_gpu_form_mt.argtypes = [_vp] * 11 + [_i32, _i32, _i32, _f64, _f64, _i32, _i32, _vp, _vp, _vp]
         L.oracle_xi_cpu_form.argtypes = [_vp] * 10 + [_i32, _i32, _i32, _f64, _f64, _f64, _f64, _vp, _vp, _vp]
         L.oracle_xi_cpu_form_mt.argtypes = [_vp] * 10 + [_i32, _i32, _i32, _f64, _f64, _f64, _f64, _i32, _i32,
                                                          _vp, _vp, _vp]
@@ -75,8 +76,11 @@ def _f1(fs, f1_list):
     return f1_list, _c(fs.slot[f1_list], np.int32)
 
 
-def xi_gpu_form(fs, nb_off, nb_idx, np_rp=50, np_rt=50, rpmax=200.0, rtmax=200.0, f1_list=None, we=None, dw=None):
-    """(cnt, w, dw) of shape [n_slots, np_rp, np_rt] with the GPU-form FP64 formula."""
+def xi_gpu_form(fs, nb_off, nb_idx, np_rp=50, np_rt=50, rpmax=200.0, rtmax=200.0, f1_list=None, we=None, dw=None,
+                threads=0):
+    """(cnt, w, dw) of shape [n_slots, np_rp, np_rt] with the GPU-form FP64 formula
+    (reference cuda_kernels.cpp:90-162).  threads > 0: the same loop over forest 1
+    with OpenMP (same bins and counts; sums in another order)."""
     L = lib()
     f1_list, f1_slot = _f1(fs, f1_list)
     ns = len(fs.slots)
@@ -86,11 +90,33 @@ def xi_gpu_form(fs, nb_off, nb_idx, np_rp=50, np_rt=50, rpmax=200.0, rtmax=200.0
     we = _c(fs.we if we is None else we, np.float64)
     dwa = _c(fs.dw if dw is None else dw, np.float64)
     nbi = _c(nb_idx if len(nb_idx) else np.zeros(1), np.int32)
-    L.oracle_xi_gpu_form(_p(_c(fs.pix_off, np.int64)), _p(_c(fs.dc, np.float64)), _p(we), _p(dwa),
-                         _p(_c(fs.ux, np.float64)), _p(_c(fs.uy, np.float64)), _p(_c(fs.uz, np.float64)),
-                         _p(_c(nb_off, np.int64)), _p(nbi), _p(f1_list), _p(f1_slot), len(f1_list),
-                         np_rp, np_rt, rpmax, rtmax, _p(cnt), _p(w), _p(d))
+    args = [_p(_c(fs.pix_off, np.int64)), _p(_c(fs.dc, np.float64)), _p(we), _p(dwa),
+            _p(_c(fs.ux, np.float64)), _p(_c(fs.uy, np.float64)), _p(_c(fs.uz, np.float64)),
+            _p(_c(nb_off, np.int64)), _p(nbi), _p(f1_list), _p(f1_slot), len(f1_list), np_rp, np_rt, rpmax, rtmax]
+    if threads:
+        L.oracle_xi_gpu_form_mt(*args, ns, int(threads), _p(cnt), _p(w), _p(d))
+    else:
+        L.oracle_xi_gpu_form(*args, _p(cnt), _p(w), _p(d))
     return cnt, w, d
+
+
+def pairs_of(fs, angmax, f1_list):
+    """Neighbour CSR (nb_off [nf+1], nb_idx) with rows only for the forests of
+    ``f1_list`` (others empty): the rule of reference forest_class.py:96 evaluated in
+    numpy per forest 1, (x1 x2 + y1 y2) + z1 z2 > cos(angmax), name1 != name2,
+    ra1 < ra2; ascending f2.  For checking a few output slots of a large survey."""
+    cm = np.cos(angmax)
+    rows = {}
+    for f1 in np.asarray(f1_list, dtype=np.int64):
+        mu = (fs.ux[f1] * fs.ux + fs.uy[f1] * fs.uy) + fs.uz[f1] * fs.uz
+        rows[int(f1)] = np.nonzero((mu > cm) & (fs.name != fs.name[f1]) & (fs.ra[f1] < fs.ra))[0].astype(np.int32)
+    counts = np.zeros(fs.nf, dtype=np.int64)
+    for f1, r in rows.items():
+        counts[f1] = len(r)
+    off = np.zeros(fs.nf + 1, dtype=np.int64)
+    np.cumsum(counts, out=off[1:])
+    idx = np.concatenate([rows[f] for f in sorted(rows)]) if rows else np.zeros(0, np.int32)
+    return off, idx.astype(np.int32)
 
 
 def xi_cpu_form(fs, nb_off, nb_idx, angmax, chiquito, np_rp=50, np_rt=50, rpmax=200.0, rtmax=200.0, f1_list=None,

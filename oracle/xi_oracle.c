@@ -313,3 +313,42 @@ EXPORT void oracle_xi_cpu_form_mt(const int64_t* pix_off, const double* dc, cons
         free(c); free(a); free(b);
     }
 }
+
+/* ------------------------------------------------ multi-core GPU form */
+/* oracle_xi_gpu_form over the f1 list with `threads` OpenMP threads (each forest 1
+ * by one thread, into private histograms summed at the end).  Bins and counts equal
+ * the sequential form; sums differ only in summation order.  Used to check the
+ * full-size workloads, where the sequential form would take minutes. */
+EXPORT void oracle_xi_gpu_form_mt(const int64_t* pix_off, const double* dc, const double* we, const double* dw,
+                                  const double* ux, const double* uy, const double* uz, const int64_t* nb_off,
+                                  const int32_t* nb_idx, const int32_t* f1_list, const int32_t* f1_slot, int32_t n_f1,
+                                  int32_t np, int32_t nt, double rpmax, double rtmax, int32_t n_slots, int32_t threads,
+                                  uint64_t* cnt, double* w, double* dwh) {
+    const int64_t tot = (int64_t)n_slots * np * nt;
+#ifdef _OPENMP
+    if (threads < 1) threads = omp_get_max_threads();
+#pragma omp parallel num_threads(threads)
+    {
+        const int t = omp_get_thread_num(), T = omp_get_num_threads();
+#else
+    {
+        const int t = 0, T = 1;
+        (void)threads;
+#endif
+        uint64_t* c = (uint64_t*)calloc((size_t)tot, sizeof(uint64_t));
+        double* a = (double*)calloc((size_t)tot, sizeof(double));
+        double* b = (double*)calloc((size_t)tot, sizeof(double));
+        for (int32_t k = t; k < n_f1; k += T)
+            oracle_xi_gpu_form(pix_off, dc, we, dw, ux, uy, uz, nb_off, nb_idx, f1_list + k, f1_slot + k, 1, np, nt,
+                               rpmax, rtmax, c, a, b);
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        for (int64_t m = 0; m < tot; ++m) {
+            cnt[m] += c[m];
+            w[m] += a[m];
+            dwh[m] += b[m];
+        }
+        free(c); free(a); free(b);
+    }
+}

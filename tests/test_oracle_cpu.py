@@ -192,3 +192,22 @@ def test_golden_fixture():
     assert np.array_equal(c, z["cnt"])
     np.testing.assert_allclose(w, z["w"], rtol=1e-13)
     np.testing.assert_allclose(d, z["dw_sum"], rtol=1e-9, atol=1e-9)
+
+
+def test_multicore_gpu_form_and_subset_rows(small_set, small_pairs):
+    """The OpenMP GPU form equals the sequential one (counts exactly, sums to 1e-12 of
+    the absolute sums), and pairs_of gives the O(N^2) rows of the forests asked for."""
+    am, (off, idx) = small_pairs
+    c0, w0, d0 = oracle.xi_gpu_form(small_set, off, idx)
+    _, aw, ad = oracle.xi_gpu_form(small_set, off, idx, we=np.abs(small_set.we), dw=np.abs(small_set.dw))
+    c1, w1, d1 = oracle.xi_gpu_form(small_set, off, idx, threads=4)
+    assert np.array_equal(c0, c1)
+    assert np.all(np.abs(w1 - w0) <= 1e-12 * aw) and np.all(np.abs(d1 - d0) <= 1e-12 * ad)
+    keep = np.nonzero(small_set.slot == small_set.slot[len(small_set.slot) // 2])[0]
+    soff, sidx = oracle.pairs_of(small_set, am, keep)
+    for f in range(small_set.nf):
+        row = sidx[soff[f]:soff[f + 1]]
+        assert np.array_equal(row, idx[off[f]:off[f + 1]] if f in set(keep) else row[:0]), f
+    c2, _, _ = oracle.xi_gpu_form(small_set, soff, sidx, threads=3)
+    k = int(small_set.slot[keep[0]])
+    assert np.array_equal(c2[k], c0[k]) and c2.sum() == c0[k].sum()
