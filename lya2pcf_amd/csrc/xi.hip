@@ -138,32 +138,49 @@ struct BandShape {
     int32_t br, bc, nbp, nbt;
 };
 
+// Geometry of a forest pair shared by every slab of it: the half-angle factors and
+// the pruning thresholds.  Dp bounds |r1 - r2| and Rt bounds r1 + r2 for pairs in
+// rows below np_hi and columns below nt_hi; Rlo = r1 + r2 below which r_perp is under
+// the band's first column (banded kernels).  The margins (1e-3 bin) dominate the FP64
+// rounding of the thresholds, so pruned pairs are out of range exactly.
+struct PairGeo {
+    double c, s, Dp, Rt, Rlo;
+};
+template <bool BANDED = false>
+__device__ __forceinline__ PairGeo pair_geo(const lya_xi_problem& p, int f1, int f2, double bp64, double bt64,
+                                            const Band& bd = Band{0u, 0u, 0u, 0u}) {
+    PairGeo g;
+    halfangles(cptr(p.ux)[f1], cptr(p.uy)[f1], cptr(p.uz)[f1], cptr(p.ux)[f2], cptr(p.uy)[f2], cptr(p.uz)[f2],
+               g.c, g.s);
+    g.c = unid(g.c);
+    g.s = unid(g.s);
+    const double np_hi = BANDED ? (double)(bd.bp0 + bd.rows) : (double)p.np_rp;
+    const double nt_hi = BANDED ? (double)(bd.bt0 + bd.cols) : (double)p.np_rt;
+    g.Dp = (np_hi + 1e-3) / (g.c * bp64);
+    g.Rt = (nt_hi + 1e-3) / (g.s * bt64);
+    g.Rlo = BANDED && bd.bt0 > 0u ? ((double)bd.bt0 - 1e-3) / (g.s * bt64) : -INFINITY;
+    return g;
+}
+
 // Forest-2 j range, within [j0, j1), that can hold an in-range pair for some
-// forest-1 pixel of the slab.  Needs both forests sorted by dc; the margins (1e-3
-// bin) dominate the FP64 rounding of the thresholds, so skipped pairs are out of
-// range exactly.
-// BANDED: pairs that land in the band [bp0, bp0 + rows) x [bt0, bt0 + cols) only.
-// r_perp = (r1 + r2) s grows with j, so the band's columns give a lower bound on j
-// as well; the row bound uses the band's last row (rows from 0 upwards, a superset).
+// forest-1 pixel of the slab [i0, ilast].  Needs both forests sorted by dc.
+// BANDED: pairs that land in the band only: r_perp = (r1 + r2) s grows with j, so the
+// band's columns give a lower bound on j as well; the row bound uses the band's last
+// row (rows from 0 upwards, a superset).
 template <bool BANDED = false>
 __device__ __forceinline__ void prune(const lya_xi_problem& p, int f1, int64_t p1, int i0, int ilast, int f2,
-                                      int64_t p2, int j0, int j1, double c, double s, double bp64, double bt64,
-                                      int lane, int& jlo, int& jhi, const Band& bd = Band{0u, 0u, 0u, 0u}) {
+                                      int64_t p2, int j0, int j1, const PairGeo& g, int lane, int& jlo, int& jhi) {
     jlo = j0;
     jhi = j1;
     if (!(cptr(p.sorted)[f1] & cptr(p.sorted)[f2])) return;
     const double r1min = cptr(p.dc)[p1 + i0];
     const double r1max = cptr(p.dc)[p1 + ilast];
-    const double np_hi = BANDED ? (double)(bd.bp0 + bd.rows) : (double)p.np_rp;
-    const double nt_hi = BANDED ? (double)(bd.bt0 + bd.cols) : (double)p.np_rt;
-    const double Dp = (np_hi + 1e-3) / (c * bp64);
-    const double Rt = (nt_hi + 1e-3) / (s * bt64);
-    const double lo = r1min - Dp;
-    const double hi = fmin(r1max + Dp, Rt - r1min);
+    const double lo = r1min - g.Dp;
+    const double hi = fmin(r1max + g.Dp, g.Rt - r1min);
     // band: r2 below this bound puts r_perp under the band's first column for every lane
-    const double lot = BANDED && bd.bt0 > 0u ? ((double)bd.bt0 - 1e-3) / (s * bt64) - r1max : -INFINITY;
+    const double lot = g.Rlo - r1max;
     if constexpr (BANDED) {
-        // the forest pair misses the band altogether: nothing to prune
+        // the slab misses the band altogether: nothing to prune
         if (j1 <= j0 || cptr(p.dc)[p2 + j1 - 1] < lot || cptr(p.dc)[p2 + j0] >= hi) {
             jlo = jhi = j0;
             return;
@@ -256,20 +273,17 @@ __device__ __forceinline__ UnitArgs unit_args(const TaskCtx& t, const Units& u, 
 template <int MODE, int PPL, bool CNT, bool POW2, bool MERGE, bool BANDED>
 __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char* lds, uint32_t hbase,
                                          uint32_t ntpad4, const TaskCtx& t, int i0, int f2, int u0, int u1,
-                                         int lane, int perm, double bp64, double bt64, const Band& bd) {
+                                         const PairGeo& g, int lane, int perm, double bp64, double bt64,
+                                         const Band& bd) {
     static_assert(!(BANDED && MODE == LYA_MODE_MIXED_F32), "mixed32 has no banded form");
     using L = Layout<MODE, CNT>;
     // rows and columns of the LDS histogram
     const int NP = BANDED ? (int)bd.rows : p.np_rp, NT = BANDED ? (int)bd.cols : p.np_rt;
     const int64_t p2 = cptr(p.pix_off)[f2];
-    double c, s;
-    halfangles(cptr(p.ux)[t.f1], cptr(p.uy)[t.f1], cptr(p.uz)[t.f1], cptr(p.ux)[f2], cptr(p.uy)[f2],
-               cptr(p.uz)[f2], c, s);
-    c = unid(c);
-    s = unid(s);
+    const double c = g.c, s = g.s;
     const int ilast = min(t.n1, i0 + 64 * PPL) - 1;
     int jlo, jhi;
-    prune<BANDED>(p, t.f1, t.p1, i0, ilast, f2, p2, u0, u1, c, s, bp64, bt64, lane, jlo, jhi, bd);
+    prune<BANDED>(p, t.f1, t.p1, i0, ilast, f2, p2, u0, u1, g, lane, jlo, jhi);
     if (jlo >= jhi) return;
 
     double r1[PPL];
@@ -777,26 +791,28 @@ __global__ void __launch_bounds__(WAVES * 64)
                 t.slot = slot;
                 t.p1 = cptr(p.pix_off)[t.f1];
                 t.n1 = (int)(cptr(p.pix_off)[t.f1 + 1] - t.p1);
+                // one neighbour per wave: the pair's geometry once, then all its units
                 constexpr int SL = 64 * PPL;
                 const Units un = units_of(t.n1, SL, split);
-                const int ntask = un.upq * (nb1 - nb0);
-                for (int k = wave; k < ntask; k += WAVES) {
-                    const int q = k / un.upq;
-                    const int sub = k - q * un.upq;
-                    const int f2 = cptr(p.nb_idx)[nb0 + q];
+                for (int q = nb0 + wave; q < nb1; q += WAVES) {
+                    const int f2 = cptr(p.nb_idx)[q];
                     const int64_t p2 = cptr(p.pix_off)[f2];
                     const int n2 = (int)(cptr(p.pix_off)[f2 + 1] - p2);
-                    UnitArgs a = unit_args(t, un, SL, sub, f2, p2, n2);
-                    // one call site per slab width (the inlined task body is large)
-                    for (; a.i0 < a.tl.n1; a.i0 += 64) {
-                        if (PPL > 1 && a.wide) {
-                            run_task<MODE, PPL, CNT, POW2, MERGE, BANDED>(p, lds, hbase, ntpad4, a.tl, a.i0, a.fu,
-                                                                          a.u0, a.u1, lane, perm, bp64, bt64, bd);
-                        } else {
-                            run_task<MODE, 1, CNT, POW2, MERGE, BANDED>(p, lds, hbase, ntpad4, a.tl, a.i0, a.fu,
-                                                                        a.u0, a.u1, lane, perm, bp64, bt64, bd);
+                    const PairGeo g = pair_geo<BANDED>(p, t.f1, f2, bp64, bt64, bd);
+                    for (int sub = 0; sub < un.upq; ++sub) {
+                        UnitArgs a = unit_args(t, un, SL, sub, f2, p2, n2);
+                        // one call site per slab width (the inlined task body is large)
+                        for (; a.i0 < a.tl.n1; a.i0 += 64) {
+                            if (PPL > 1 && a.wide) {
+                                run_task<MODE, PPL, CNT, POW2, MERGE, BANDED>(p, lds, hbase, ntpad4, a.tl, a.i0, a.fu,
+                                                                              a.u0, a.u1, g, lane, perm, bp64, bt64,
+                                                                              bd);
+                            } else {
+                                run_task<MODE, 1, CNT, POW2, MERGE, BANDED>(p, lds, hbase, ntpad4, a.tl, a.i0, a.fu,
+                                                                            a.u0, a.u1, g, lane, perm, bp64, bt64, bd);
+                            }
+                            if (!a.tail) break;
                         }
-                        if (!a.tail) break;
                     }
                 }
                 if constexpr (MODE == LYA_MODE_MIXED_F32) {
@@ -1238,14 +1254,11 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
                 const int f2 = cptr(p.nb_idx)[nb0 + q];
                 const int64_t p2 = cptr(p.pix_off)[f2];
                 const int n2 = (int)(cptr(p.pix_off)[f2 + 1] - p2);
-                double c, s;
-                halfangles(cptr(p.ux)[f1], cptr(p.uy)[f1], cptr(p.uz)[f1], cptr(p.ux)[f2], cptr(p.uy)[f2],
-                           cptr(p.uz)[f2], c, s);
-                c = unid(c);
-                s = unid(s);
+                const PairGeo g = pair_geo(p, f1, f2, bp64, bt64);
+                const double c = g.c, s = g.s;
                 const int ilast = min(n1, slab * SL + SL) - 1;
                 int jlo, jhi;
-                prune(p, f1, p1, slab * SL, ilast, f2, p2, 0, n2, c, s, bp64, bt64, lane, jlo, jhi);
+                prune(p, f1, p1, slab * SL, ilast, f2, p2, 0, n2, g, lane, jlo, jhi);
                 if (jlo >= jhi) continue;
                 int i[PPL];
                 bool valid[PPL];
@@ -1405,16 +1418,15 @@ __global__ void work_stats_kernel(const lya_xi_problem p, int split, unsigned lo
             const int f2 = p.nb_idx[q];
             const int64_t p2 = p.pix_off[f2];
             const int n2 = (int)(p.pix_off[f2 + 1] - p2);
-            double c, s;
-            halfangles(p.ux[t.f1], p.uy[t.f1], p.uz[t.f1], p.ux[f2], p.uy[f2], p.uz[f2], c, s);
+            const PairGeo g = pair_geo(p, t.f1, f2, bp64, bt64);
             for (int sub = 0; sub < un.upq; ++sub) {
                 UnitArgs a = unit_args(t, un, SL, sub, f2, p2, n2);
                 for (; a.i0 < a.tl.n1; a.i0 += 64) {
                     const int w = (PPL > 1 && a.wide) ? SL : 64;
                     const int ilast = min(a.tl.n1, a.i0 + w) - 1;
                     int jlo, jhi;
-                    prune(p, a.tl.f1, a.tl.p1, a.i0, ilast, a.fu, p.pix_off[a.fu], a.u0, a.u1, c, s, bp64, bt64,
-                          lane, jlo, jhi);
+                    prune(p, a.tl.f1, a.tl.p1, a.i0, ilast, a.fu, p.pix_off[a.fu], a.u0, a.u1, g, lane, jlo,
+                          jhi);
                     const unsigned long long span = (unsigned long long)max(0, jhi - jlo);
                     slots += (unsigned long long)w * span;
                     valid += (unsigned long long)(ilast - a.i0 + 1) * span;
