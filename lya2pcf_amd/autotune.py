@@ -8,6 +8,8 @@ Search space (all shapes give identical bins):
     waves        4, 8, 16  -> waves per workgroup
     lane_map     0, 1, 2   -> lane order of forest-1 pixels (consecutive, by 2, by 4)
     merge        0, 1      -> sum runs of equal bins per block before the LDS atomics
+    tail_max     -1, 32, 48, 63 -> longest forest-1 tail walked against forest 2 in lanes
+                             (-1: slabs of 64*px_per_lane pixels only)
   walk 1 (lanes walk diagonals, FP64 only):
     waves        8, 16
 Two replicas exist for 8-wave workgroups only; four would not leave room in the
@@ -35,9 +37,9 @@ CACHE = os.path.join(HERE, "tuned.json")
 PROFILES = os.path.join(os.path.dirname(HERE), "profiles")
 
 SPACE = {"px_per_lane": (1, 2, 4), "replicas": (1,), "row_pad": (1,), "waves": (8, 16), "lane_map": (1, 2),
-         "merge": (0, 1)}
+         "merge": (0, 1), "tail_max": (32, 48)}
 FULL_SPACE = {"px_per_lane": (1, 2, 4), "replicas": (1, 2), "row_pad": (0, 1), "waves": (4, 8, 16),
-              "lane_map": (0, 1, 2), "merge": (0, 1)}
+              "lane_map": (0, 1, 2), "merge": (0, 1), "tail_max": (-1, 32, 48, 63)}
 WALK_SPACE = {"walk": (1,), "waves": (8, 16)}
 SHAPE_KEYS = ("px_per_lane", "waves", "replicas", "row_pad", "lane_map", "walk", "merge", "tail_max")
 
