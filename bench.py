@@ -52,6 +52,29 @@ def time_graph(g, eng, steps, warmup, torch):
     return wall, ev
 
 
+def lds_atomic_rates(torch):
+    """LDS atomic adds per CU per clock for uint32, float and double (lya_debug_lds_atomic):
+    why FP32 LDS partials do not pay on this part."""
+    from lya2pcf_amd import _lib
+    props = torch.cuda.get_device_properties(0)
+    cus = props.multi_processor_count
+    clock_hz = 2.4e9  # MI355X peak engine clock
+    blocks, iters = cus * 4, 4096
+    out = {}
+    for t, name, dt in ((0, "u32", torch.int32), (1, "f32", torch.float32), (2, "f64", torch.float64)):
+        buf = torch.empty(blocks, dtype=dt, device="cuda")
+        for rep in range(2):  # the first launch loads the kernel
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            _lib.check(_lib.lib().lya_debug_lds_atomic(t, blocks, iters, _lib.ptr(buf), _lib.stream_ptr()))
+            e1.record()
+            torch.cuda.synchronize()
+        sec = e0.elapsed_time(e1) / 1e3
+        out[name] = blocks * 512.0 * iters / sec / cus / clock_hz
+    log("LDS atomic adds per CU per clock: %s" % {k: round(v, 2) for k, v in out.items()})
+    return out
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -387,6 +410,7 @@ def main():
         line["mixed32"] = {"value": P / out["mixed32"]["wall"], "ms_per_step": out["mixed32"]["wall"] * 1e3,
                            "counts_bit_exact": bool(np.array_equal(c64, c3)),
                            "xi_total_max_rel_err": float(np.max(np.abs(X3 - X64)[sel] / np.abs(X64[sel])))}
+    line["lds_atomic_adds_per_cu_clock"] = lds_atomic_rates(torch)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log("cpu baseline: %d threads, ~%.2g pixel pairs" % (threads, args.cpu_pairs))

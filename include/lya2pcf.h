@@ -295,6 +295,10 @@ int lya_distortion_mirror(double* D, int32_t nb, void* stream);
 int lya_debug_halfangles(const double* cos12, double* c, double* s, int32_t n, void* stream);
 /* Plain device copy of n bytes (profiler calibration of HBM byte counters). */
 int lya_copy_bytes(const void* src, void* dst, int64_t n, void* stream);
+/* LDS atomic-add rate probe: `blocks` workgroups of 512 threads, each lane adding 1
+ * `iters` times to its own LDS entry; type 0 uint32, 1 float, 2 double.  out holds
+ * `blocks` elements of that type.  The caller times the launch. */
+int lya_debug_lds_atomic(int32_t type, int32_t blocks, int32_t iters, void* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -96,6 +96,7 @@ def lib():
     L.lya_segment_sum.argtypes = [_vp, _vp, _i32, _vp, _vp]
     L.lya_debug_halfangles.argtypes = [_vp, _vp, _vp, _i32, _vp]
     L.lya_copy_bytes.argtypes = [_vp, _vp, _i64, _vp]
+    L.lya_debug_lds_atomic.argtypes = [_i32, _i32, _i32, _vp, _vp]
     L.lya_build_tiles.argtypes = [_i64, _vp, _vp, _i64, _vp, _vp]
     L.lya_pairs_device_scratch_bytes.argtypes = [_i32, _f64, _vp]
     L.lya_build_pairs_device.argtypes = [_i32, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _vp, _i64, _vp, _vp]
@@ -118,6 +119,7 @@ EXPORTS = ("lya_last_error", "lya_version", "lya_build_pairs", "lya_build_work",
            "lya_xi_adjoint",
            "lya_distortion_scratch_bytes", "lya_distortion_run", "lya_distortion_run_noalloc",
            "lya_distortion_mirror", "lya_segment_sum", "lya_debug_halfangles", "lya_copy_bytes",
+           "lya_debug_lds_atomic",
            "lya_build_tiles", "lya_pairs_device_scratch_bytes", "lya_build_pairs_device",
            "lya_items_device_scratch_bytes", "lya_build_items_device")
 

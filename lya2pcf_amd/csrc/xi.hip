@@ -179,12 +179,11 @@ __device__ __forceinline__ void prune(const lya_xi_problem& p, int f1, int64_t p
     const double hi = fmin(r1max + g.Dp, g.Rt - r1min);
     // band: r2 below this bound puts r_perp under the band's first column for every lane
     const double lot = g.Rlo - r1max;
-    if constexpr (BANDED) {
-        // the slab misses the band altogether: nothing to prune
-        if (j1 <= j0 || cptr(p.dc)[p2 + j1 - 1] < lot || cptr(p.dc)[p2 + j0] >= hi) {
-            jlo = jhi = j0;
-            return;
-        }
+    // the whole range below lo (or lot) or from hi on: empty, without the counting pass
+    if (j1 <= j0 || cptr(p.dc)[p2 + j1 - 1] <= lo || cptr(p.dc)[p2 + j0] >= hi ||
+        (BANDED && cptr(p.dc)[p2 + j1 - 1] < lot)) {
+        jlo = jhi = j0;
+        return;
     }
     int clo = 0, chi = 0;
     const int jr = j0 + ((j1 - j0 + 63) & ~63);
@@ -1477,6 +1476,22 @@ __global__ void halfangle_kernel(const double* cos12, double* c, double* s, int 
     s[i] = sqrt(0.5 * (1.0 - x));
 }
 
+// LDS atomic-add rate: every lane adds to its own entry of a 4096-entry table, 64
+// entries apart from one add to the next (indices stay in [0, 4096)).
+template <class T>
+__global__ void __launch_bounds__(512) lds_atomic_probe(T* out, int iters) {
+    __shared__ T h[4096];
+    for (int i = threadIdx.x; i < 4096; i += blockDim.x) h[i] = (T)0;
+    __syncthreads();
+    uint32_t a = threadIdx.x & 4095u;
+    for (int k = 0; k < iters; ++k) {
+        __hip_atomic_fetch_add(h + a, (T)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        a = (a + 64u) & 4095u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = h[threadIdx.x];
+}
+
 __global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
@@ -1861,6 +1876,16 @@ extern "C" int lya_debug_halfangles(const double* cos12, double* c, double* s, i
     hipLaunchKernelGGL(halfangle_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, cos12, c, s, n);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? LYA_OK : lya_fail(LYA_ERR_HIP, "halfangle launch: %s", hipGetErrorString(e));
+}
+
+extern "C" int lya_debug_lds_atomic(int32_t type, int32_t blocks, int32_t iters, void* out, void* stream) {
+    if (blocks < 1 || iters < 0 || !out || type < 0 || type > 2) return lya_fail(LYA_ERR_ARG, "lds probe: bad arguments");
+    hipStream_t st = (hipStream_t)stream;
+    if (type == 0) hipLaunchKernelGGL(lds_atomic_probe<uint32_t>, dim3(blocks), dim3(512), 0, st, (uint32_t*)out, iters);
+    if (type == 1) hipLaunchKernelGGL(lds_atomic_probe<float>, dim3(blocks), dim3(512), 0, st, (float*)out, iters);
+    if (type == 2) hipLaunchKernelGGL(lds_atomic_probe<double>, dim3(blocks), dim3(512), 0, st, (double*)out, iters);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? LYA_OK : lya_fail(LYA_ERR_HIP, "lds probe launch: %s", hipGetErrorString(e));
 }
 
 extern "C" int lya_copy_bytes(const void* src, void* dst, int64_t n, void* stream) {

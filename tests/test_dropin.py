@@ -300,3 +300,47 @@ def test_distortion_multiple_data(tmp_path):
     ok = w0 > 0
     assert np.array_equal(np.isfinite(D).all(1), ok)
     assert np.max(np.abs(D[ok] - E[ok])) <= 1e-10 * np.abs(E[ok]).max()
+
+
+@pytest.mark.gpu
+def test_postprocessing_of_gpu_outputs(tmp_path):
+    """End to end: 2pla.py --gpu and distortion.py write their files, the
+    post-processing consumer (reference post_processing.py:122-217) reads them, and
+    xi, the smoothed covariance and the COR table equal the same consumer run on
+    oracle histograms and the oracle distortion."""
+    import random
+    from lya2pcf_amd import distortion as dist, fits, postproc
+    s = _workspace(tmp_path, n=120, seed=12, length_range=(40, 120))
+    _run(tmp_path, "2pla.py", "--gpu")
+    _run(tmp_path, "distortion.py", "--excluded", "0.8")
+    out = tmp_path / "out"
+    corr, err = postproc.main(["--corr-dir", str(out)])
+    # the same consumer on oracle outputs
+    ref = tmp_path / "ref"
+    ref.mkdir()
+    fs = forests.ForestSet.from_survey(s)
+    am = forests.angmax_from(fs, 200.0)
+    off, idx = oracle.pairs(fs, am)
+    c0, w0, d0 = oracle.xi_gpu_form(fs, off, idx)
+    for k, pix in enumerate(fs.slots):
+        np.save(ref / ("2d_histogram_pixel_%d" % pix), (w0[k], d0[k]))
+    pairs = dist.subsample(fs, off, idx, range(fs.nf), 0.8, rng=random.Random(1))
+    D0, wA0 = oracle.distortion(fs, pairs)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        np.save(ref / "distortion", D0 / wA0[:, None])
+    corr0, err0 = postproc.main(["--corr-dir", str(ref)])
+    x, x0 = np.load(out / "correlation_2d.npy"), np.load(ref / "correlation_2d.npy")
+    assert x.shape == (50, 50) and np.abs(x0).max() > 0
+    np.testing.assert_allclose(x, x0, rtol=1e-9, atol=1e-10 * np.abs(x0).max())
+    C, C0 = np.load(out / "covariance.npy"), np.load(ref / "covariance.npy")
+    np.testing.assert_allclose(C, C0, rtol=1e-8, atol=1e-9 * np.nanmax(np.abs(C0)))
+    t = fits.by_name(fits.read(out / "correlation.out.gz"), "COR")
+    t0 = fits.by_name(fits.read(ref / "correlation.out.gz"), "COR")
+    assert t.header["NP"] == t0.header["NP"] == 50
+    np.testing.assert_allclose(t.data["DA"], t0.data["DA"], rtol=1e-9, atol=1e-10 * np.abs(x0).max())
+    np.testing.assert_allclose(t.data["CO"], t0.data["CO"], rtol=1e-8, atol=1e-9 * np.nanmax(np.abs(C0)))
+    dm, dm0 = t.data["DM"], t0.data["DM"]
+    ok = np.isfinite(dm0).all(1)
+    assert np.array_equal(np.isfinite(dm).all(1), ok) and ok.any()
+    assert np.max(np.abs(dm[ok] - dm0[ok])) <= 1e-10 * np.abs(dm0[ok]).max()
+    assert np.array_equal(t.data["RP"], t0.data["RP"]) and np.array_equal(t.data["RT"], t0.data["RT"])
