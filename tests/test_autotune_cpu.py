@@ -10,7 +10,9 @@ def test_full_space_covers_tiles_and_layouts():
     assert {(c["row_pad"], c["replicas"]) for c in full} == {(0, 1), (1, 1), (0, 2), (1, 2)}
     # two replicas are built for 8-wave workgroups only
     assert all(c["waves"] == 8 for c in full if c["replicas"] == 2)
-    assert len(full) == 3 * 2 * 3 * 3 * 2 + 3 * 2 * 3 * 2
+    # forest-1 tails walked against forest 2 in lanes: off, or up to 32, 48, 63 pixels
+    assert {c["tail_max"] for c in full} == {-1, 32, 48, 63}
+    assert len(full) == (3 * 2 * 3 * 3 * 2 + 3 * 2 * 3 * 2) * 4
 
 
 def test_default_space_inside_full_space():
