@@ -184,8 +184,9 @@ typedef struct lya_xi_config {
                               /* 64*px_per_lane pixels only, the last one partial      */
 } lya_xi_config;
 
-/* Largest mixed-mode error bound for these data (host): returns tau in bin units. */
-int lya_xi_tau(const lya_xi_problem* host_view_of_sizes, double max_abs_off, double max_c_binner,
+/* Largest mixed-mode error bound for these data (host): returns tau in bin units.
+ * max_abs_off = largest |dc - base| (the FP32 offsets), max_dc = largest dc. */
+int lya_xi_tau(const lya_xi_problem* host_view_of_sizes, double max_abs_off, double max_dc, double max_c_binner,
                double max_s_binner, double* tau);
 
 /* Zero outputs and queue, then launch the persistent kernel on `stream`.

@@ -78,7 +78,7 @@ def lib():
     L.lya_version.restype = C.c_int
     L.lya_build_pairs.argtypes = [_i32, _vp, _vp, _vp, _vp, _vp, _f64, _vp, _vp, _i64, _vp]
     L.lya_build_work.argtypes = [_i32, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp]
-    L.lya_xi_tau.argtypes = [C.POINTER(XiProblem), _f64, _f64, _f64, C.POINTER(_f64)]
+    L.lya_xi_tau.argtypes = [C.POINTER(XiProblem), _f64, _f64, _f64, _f64, C.POINTER(_f64)]
     L.lya_xi_run.argtypes = [C.POINTER(XiProblem), C.POINTER(XiConfig), _vp]
     L.lya_xi_graph_create.argtypes = [C.POINTER(XiProblem), C.POINTER(XiConfig), _vp, C.POINTER(_vp)]
     L.lya_graph_launch.argtypes = [_vp, _vp]

@@ -1778,25 +1778,25 @@ extern "C" int lya_graph_destroy(lya_graph_t g) {
     return LYA_OK;
 }
 
-extern "C" int lya_xi_tau(const lya_xi_problem* p, double max_abs_off, double max_c_binner, double max_s_binner,
-                          double* tau) {
+extern "C" int lya_xi_tau(const lya_xi_problem* p, double max_abs_off, double max_dc, double max_c_binner,
+                          double max_s_binner, double* tau) {
     // Bound on |(FP32 scaled coordinate) - (FP64 scaled coordinate)| for pairs
     // whose coordinate is below np + 3 bins (larger ones are out of range in
     // both evaluations).  Terms, each a unit roundoff u32 = 2^-24 times a
     // magnitude bound, follow the mixed-mode arithmetic of run_task:
     //   rounding of P1 (or T1), of the fma result, of |v| + tau, the FP32
     //   representation of off32 and of c*binner (or s*binner), plus the FP64
-    //   roundings of the reference formula.  r1 - base2 and r1 - r2 are exact in
-    //   FP64 (Sterbenz).  The kernel's band is exact for an error e <= tau: a pair
+    //   roundings of the reference formula and of r1 + base2 (at most 2 max_dc)
+    //   in T1.  r1 - base2 and r1 - r2 are exact in FP64 (Sterbenz).  The kernel's band is exact for an error e <= tau: a pair
     //   whose FP32 bin differs from its FP64 bin has its shifted FP32 coordinate
     //   within tau + e above an edge and is flagged (fract < 2 tau).  The bound
     //   alone would do; a margin of 25 % is kept.
-    if (!p || !tau || !(max_abs_off >= 0)) return lya_fail(LYA_ERR_ARG, "lya_xi_tau: bad arguments");
+    if (!p || !tau || !(max_abs_off >= 0) || !(max_dc >= 0)) return lya_fail(LYA_ERR_ARG, "lya_xi_tau: bad arguments");
     const double u32 = 1.0 / 16777216.0, u64 = 1.0 / 9007199254740992.0;
     auto bound = [&](double nb, double kb) {
         const double q = nb + 3.0;        // largest relevant coordinate
         const double m = max_abs_off * kb;  // largest |off * k|
-        return u32 * (2.0 * q + (q + m) + 2.0 * m) + 8.0 * u64 * (q + m + 1e4 * kb);
+        return u32 * (2.0 * q + (q + m) + 2.0 * m) + 8.0 * u64 * (q + m + 2.0 * max_dc * kb);
     };
     const double t = 1.25 * std::max(bound(p->np_rp, max_c_binner), bound(p->np_rt, max_s_binner));
     *tau = t;

@@ -325,7 +325,8 @@ class XiEngine:
         p = XiProblem()
         p.np_rp, p.np_rt = self.np_rp, self.np_rt
         t = C.c_double(0.0)
-        check(lib().lya_xi_tau(C.byref(p), self.forests.max_abs_off, self.np_rp / self.rpmax,
+        max_dc = float(np.max(self.forests.dc)) if len(self.forests.dc) else 0.0
+        check(lib().lya_xi_tau(C.byref(p), self.forests.max_abs_off, max_dc, self.np_rp / self.rpmax,
                                self.np_rt / self.rtmax, C.byref(t)))
         # round up to the next float32 so the kernel band is never narrower than the bound
         tf = np.float32(t.value)

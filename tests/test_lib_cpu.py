@@ -68,8 +68,13 @@ def test_tau_bound_small():
     p = _lib.XiProblem()
     p.np_rp = p.np_rt = 50
     t = C.c_double()
-    _lib.check(_lib.lib().lya_xi_tau(C.byref(p), 210.0, 0.25, 0.25, C.byref(t)))
+    _lib.check(_lib.lib().lya_xi_tau(C.byref(p), 210.0, 6000.0, 0.25, 0.25, C.byref(t)))
     assert 1e-6 < t.value < 1e-4
+    # the FP64 term grows with the largest distance; the FP32 terms dominate
+    t2 = C.c_double()
+    _lib.check(_lib.lib().lya_xi_tau(C.byref(p), 210.0, 1e9, 0.25, 0.25, C.byref(t2)))
+    assert t2.value > t.value and t2.value < 1.01 * t.value + 1.25 * 8 * 2.0 ** -53 * 2e9 * 0.25
+    assert _lib.lib().lya_xi_tau(C.byref(p), 210.0, -1.0, 0.25, 0.25, C.byref(t2)) == _lib.LYA_ERR_ARG
 
 
 def test_missing_library_fails_loudly(tmp_path):
@@ -133,7 +138,7 @@ def test_mixed_edge_band_flags_every_misbinned_pair():
     p = _lib.XiProblem()
     p.np_rp = p.np_rt = 50
     t = C.c_double()
-    _lib.check(_lib.lib().lya_xi_tau(C.byref(p), max_off, binner, binner, C.byref(t)))
+    _lib.check(_lib.lib().lya_xi_tau(C.byref(p), max_off, 5600.0 + max_off, binner, binner, C.byref(t)))
     tau = np.float32(t.value)
     if float(tau) < t.value:
         tau = np.nextafter(tau, np.float32(1))
