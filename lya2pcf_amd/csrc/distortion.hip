@@ -29,13 +29,19 @@
 //      numbers 0..nact-1 in bin order.  No per-pair bin table is kept: bins are
 //      recomputed where needed, which leaves the LDS to the Z chunks.
 //   2. Z is built in chunks of at most 128 rows in LDS, zs[kk][b]: one thread per
-//      pixel pair of the chunk recomputes its bin and adds its weight (LDS atomics).
+//      pixel pair of the chunk (two per step of the loop) recomputes its bin and adds
+//      its weight (LDS atomics).
 //      The four rank rows follow from the eta21, eta31 rows of the j chunks
 //      (S00[A] = O1 sum_j w_j eta21[A][j], ...) without another pass over the pairs;
 //      they are kept in a small per-workgroup global scratch.
-//   3. The upper triangle of D_pair is cut into 16x16 blocks dealt round-robin to
-//      the waves; each block is one v_mfma_f64_16x16x4f64 accumulator (FP64 matrix
-//      cores, four Z rows per instruction), updated by every chunk.
+//   3. The upper triangle of D_pair is cut into 16x16 blocks, each one
+//      v_mfma_f64_16x16x4f64 accumulator (FP64 matrix cores, four Z rows per
+//      instruction) updated by every chunk.  A pass over Z gives every wave the same
+//      number NQ <= 10 of consecutive blocks (NQ a compile-time constant per pass, so
+//      that the accumulators stay in registers and the operand loads of the next four
+//      rows run ahead of the MFMAs); larger pairs take several passes.  Z rows sit
+//      ldz = npad + 4 doubles apart, which staggers the four rows an MFMA operand load
+//      reads across the LDS banks.
 //   4. The blocks go into the upper triangle of D (FP64 global atomics, act is
 //      ascending so row <= column); lya_distortion_mirror() fills the lower
 //      triangle once accumulation is over.
@@ -46,6 +52,7 @@
 #include <cmath>
 #include <cstdint>
 #include <mutex>
+#include <type_traits>
 
 #include "../../include/lya2pcf.h"
 #include "errors.h"
@@ -61,8 +68,9 @@ typedef double double4v __attribute__((ext_vector_type(4)));
 constexpr int DWAVES = 8;
 constexpr int DTHREADS = DWAVES * 64;
 constexpr int BS = 16;       // D_pair block edge: one v_mfma_f64_16x16x4f64 accumulator
-constexpr int BPW = 16;      // blocks per wave per pass over Z
+constexpr int BPW = 10;      // blocks per wave per pass over Z (at most; 80 per pass)
 constexpr int KC_MAX = 128;  // Z rows per LDS chunk
+constexpr int ZSKEW = 4;     // extra doubles per Z row in LDS: rows k and k+1 start 32 bytes apart in the banks
 constexpr size_t LDS_STATIC = 256;         // the kernel's static LDS, rounded up
 
 template <class T>
@@ -119,18 +127,33 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
     const int tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
     const double bp64 = (double)NP / a.rpmax, bt64 = (double)NT / a.rtmax;
 
-    // visits (r, c) for r < nr, c < nc in row-major order, strided over the workgroup
-    auto grid = [&](int nr, int nc, auto&& f) {
+    // visits the cells (r, c), r < nr, c < nc, in row-major order strided over the
+    // workgroup, two per call: f(rA, cA, rB, cB, hasB) with cell B = cell A + DTHREADS
+    // (B repeats A past the end), so that the bins of both are computed before either
+    // is stored
+    auto grid2 = [&](int nr, int nc, auto&& f) {
         if (nr <= 0 || nc <= 0) return;
-        int r = tid / nc, c = tid - r * nc;
-        const int dr = DTHREADS / nc, dc = DTHREADS - dr * nc;
-        while (r < nr) {
-            f(r, c);
-            r += dr;
-            c += dc;
-            if (c >= nc) {
-                c -= nc;
-                ++r;
+        int ra = tid / nc, ca = tid - ra * nc;
+        int rb = ra + DTHREADS / nc, cb = ca + (DTHREADS - (DTHREADS / nc) * nc);
+        if (cb >= nc) {
+            cb -= nc;
+            ++rb;
+        }
+        const int dr = (2 * DTHREADS) / nc, dc = 2 * DTHREADS - dr * nc;
+        while (ra < nr) {
+            const bool hb = rb < nr;
+            f(ra, ca, hb ? rb : ra, hb ? cb : ca, hb);
+            ra += dr;
+            ca += dc;
+            if (ca >= nc) {
+                ca -= nc;
+                ++ra;
+            }
+            rb += dr;
+            cb += dc;
+            if (cb >= nc) {
+                cb -= nc;
+                ++rb;
             }
         }
     };
@@ -181,9 +204,10 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
         }
         for (int b = tid; b < NB; b += DTHREADS) cmap[b] = -1;
         __syncthreads();
-        grid(n2, n1, [&](int j, int i) {
-            const int g = gbin(i, j);
-            if (g >= 0) cmap[g] = 1;
+        grid2(n2, n1, [&](int ja, int ia, int jb, int ib, bool hb) {
+            const int ga = gbin(ia, ja), gb = gbin(ib, jb);
+            if (ga >= 0) cmap[ga] = 1;
+            if (hb && gb >= 0) cmap[gb] = 1;
         });
         __syncthreads();
         // compaction in bin order (wave 0)
@@ -211,8 +235,9 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
         const int nact = uni(s_nact);
         if (nact == 0) continue;
         // the LDS chunk buffer must hold two Z rows of the pair's active bins (padded to
-        // the block edge); a pair with more active bins is skipped and counted in queue[2]
-        if ((a.zcap / (((nact + BS - 1) / BS) * BS)) < 2) {
+        // the block edge, plus the row skew); a pair with more active bins is skipped and
+        // counted in queue[2]
+        if (a.zcap / (((nact + BS - 1) / BS) * BS + ZSKEW) < 2) {
             if (tid == 0) atomicAdd(a.skip + 1, 1u);
             continue;
         }
@@ -225,41 +250,53 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
         // ---- 2. D_pair = Z diag(v) Z^T over the upper triangle ----------------------
         const int NBK = (nact + BS - 1) / BS;
         const int npad = NBK * BS;
+        const int ldz = npad + ZSKEW;   // LDS row stride of the Z chunk
         const int nblk = NBK * (NBK + 1) / 2;
-        // chunk rows in zs[0 .. kc_max npad); the four rank rows T00, T01, T10, T11 (see the
+        // chunk rows in zs[0 .. kc_max ldz); the four rank rows T00, T01, T10, T11 (see the
         // rank chunks below) in the workgroup's global scratch, T[q npad + b]
         double* T = a.tscr + (int64_t)blockIdx.x * 4 * a.npad_max;
-        const int kc_max = min(KC_MAX, (a.zcap / npad) & ~1);
+        const int kc_max = min(KC_MAX, (a.zcap / ldz) & ~1);
         const int nchunk_j = (2 * n2 + kc_max - 1) / kc_max, nchunk_i = (2 * n1 + kc_max - 1) / kc_max;
         const int nrank = kc_max >= 4 ? 1 : 2;   // rank chunks of 4 rows, or of 2
         const int nchunk = nchunk_j + nchunk_i + nrank;
-        for (int t0 = 0; t0 < nblk; t0 += DWAVES * BPW) {
-            // this wave's 16x16 blocks t0 + q*DWAVES + wave (q < nbw), as (row block,
-            // column block >= row block); the lane's A and B operand column in the block
-            int boa[BPW], bob[BPW];
-            const int nbw = max(0, min(BPW, (nblk - t0 - wave + DWAVES - 1) / DWAVES));
+        // clears rows 0 .. kc-1 of the Z chunk (ldz is even and zs 16-byte aligned)
+        auto zero_rows = [&](int kc) {
+            double2* z2 = (double2*)zs;
+            for (int e = tid; e < (kc * ldz) / 2; e += DTHREADS) z2[e] = make_double2(0.0, 0.0);
+        };
+
+        // One pass over Z for the blocks t0 .. t0 + DWAVES*NQ - 1: every wave takes NQ
+        // consecutive blocks (the same NQ for all waves, a compile-time count, so that the
+        // accumulators stay in registers and the operand loads can run ahead of the MFMAs).
+        auto pass = [&](auto NQC, const int t0) {
+            constexpr int NQ = decltype(NQC)::value;
+            // this wave's blocks t0 + wave NQ + q as (row block, column block >= row block);
+            // the lane's A and B operand column; blocks past nblk are computed on row 0
+            // of the chunk and never written
+            uint32_t boa[NQ], bob[NQ];
+            bool bval[NQ];
 #pragma unroll
-            for (int q = 0; q < BPW; ++q) {
-                int t = t0 + q * DWAVES + wave;
+            for (int q = 0; q < NQ; ++q) {
+                int t = t0 + wave * NQ + q;
+                bval[q] = t < nblk;
                 int ba = 0;
-                while (q < nbw && t >= NBK - ba) { t -= NBK - ba; ++ba; }
-                // blocks past nbw read row 0 of the chunk and are never accumulated
-                boa[q] = q < nbw ? ba * BS + (lane & 15) : (lane & 15);
-                bob[q] = q < nbw ? (ba + t) * BS + (lane & 15) : (lane & 15);
+                while (bval[q] && t >= NBK - ba) { t -= NBK - ba; ++ba; }
+                boa[q] = (uint32_t)((bval[q] ? ba * BS : 0) + (lane & 15));
+                bob[q] = (uint32_t)((bval[q] ? (ba + t) * BS : 0) + (lane & 15));
             }
-            double4v acc[BPW];
+            double4v acc[NQ];
 #pragma unroll
-            for (int q = 0; q < BPW; ++q) acc[q] = (double4v){0.0, 0.0, 0.0, 0.0};
+            for (int q = 0; q < NQ; ++q) acc[q] = (double4v){0.0, 0.0, 0.0, 0.0};
 
             for (int ch = 0; ch < nchunk; ++ch) {
-                // build the chunk's rows of Z in zb[kk][b] and their weights in vk[kk]
+                // build the chunk's rows of Z in zs[kk][b] and their weights in vk[kk]
                 int kc;
                 if (ch < nchunk_j) {
                     // rows (eta21, eta31) of forest-2 pixels j0 .. j0 + kc/2
                     const int j0 = ch * (kc_max / 2);
                     const int nj = min(kc_max / 2, n2 - j0);
                     kc = 2 * nj;
-                    for (int e = tid; e < kc * npad; e += DTHREADS) zs[e] = 0.0;
+                    zero_rows(kc);
                     if (ch == 0 && t0 == 0)
                         for (int e = tid; e < 4 * npad; e += DTHREADS) T[e] = 0.0;
                     if (tid < kc) {
@@ -267,11 +304,15 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
                         vk[tid] = (tid & 1) ? -wj * L1 : -wj * O1;
                     }
                     __syncthreads();
-                    grid(nj, n1, [&](int jj, int i) {
-                        const int b = cbin(i, j0 + jj);
-                        if (b >= 0) {
-                            atomicAdd(zs + (2 * jj) * npad + b, sw1[i] * iO1);
-                            atomicAdd(zs + (2 * jj + 1) * npad + b, swd1[i] * iL1);
+                    grid2(nj, n1, [&](int ja, int ia, int jb, int ib, bool hb) {
+                        const int ba = cbin(ia, j0 + ja), bb = cbin(ib, j0 + jb);
+                        if (ba >= 0) {
+                            atomicAdd(zs + (2 * ja) * ldz + ba, sw1[ia] * iO1);
+                            atomicAdd(zs + (2 * ja + 1) * ldz + ba, swd1[ia] * iL1);
+                        }
+                        if (hb && bb >= 0) {
+                            atomicAdd(zs + (2 * jb) * ldz + bb, sw1[ib] * iO1);
+                            atomicAdd(zs + (2 * jb + 1) * ldz + bb, swd1[ib] * iL1);
                         }
                     });
                 } else if (ch < nchunk_j + nchunk_i) {
@@ -279,17 +320,21 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
                     const int i0 = (ch - nchunk_j) * (kc_max / 2);
                     const int ni = min(kc_max / 2, n1 - i0);
                     kc = 2 * ni;
-                    for (int e = tid; e < kc * npad; e += DTHREADS) zs[e] = 0.0;
+                    zero_rows(kc);
                     if (tid < kc) {
                         const double wi = sw1[i0 + (tid >> 1)];
                         vk[tid] = (tid & 1) ? -wi * L2 : -wi * O2;
                     }
                     __syncthreads();
-                    grid(n2, ni, [&](int j, int ii) {
-                        const int b = cbin(i0 + ii, j);
-                        if (b >= 0) {
-                            atomicAdd(zs + (2 * ii) * npad + b, sw2[j] * iO2);
-                            atomicAdd(zs + (2 * ii + 1) * npad + b, swd2[j] * iL2);
+                    grid2(n2, ni, [&](int ja, int ia, int jb, int ib, bool hb) {
+                        const int ba = cbin(i0 + ia, ja), bb = cbin(i0 + ib, jb);
+                        if (ba >= 0) {
+                            atomicAdd(zs + (2 * ia) * ldz + ba, sw2[ja] * iO2);
+                            atomicAdd(zs + (2 * ia + 1) * ldz + ba, swd2[ja] * iL2);
+                        }
+                        if (hb && bb >= 0) {
+                            atomicAdd(zs + (2 * ib) * ldz + bb, sw2[jb] * iO2);
+                            atomicAdd(zs + (2 * ib + 1) * ldz + bb, swd2[jb] * iL2);
                         }
                     });
                 } else {
@@ -300,7 +345,10 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
                     // j chunks.  S_q[A] S_q[B] v_q becomes T_q[A] T_q[B] (v_q times the factors).
                     kc = nrank == 1 ? 4 : 2;
                     const int r0 = (ch - nchunk_j - nchunk_i) * 2;
-                    for (int e = tid; e < kc * npad; e += DTHREADS) zs[e] = T[r0 * npad + e];
+                    for (int e = tid; e < kc * npad; e += DTHREADS) {
+                        const int r = e / npad, b = e - r * npad;
+                        zs[r * ldz + b] = T[(r0 + r) * npad + b];
+                    }
                     if (tid < kc) {
                         const int q = r0 + tid;
                         vk[tid] = (q < 2 ? O1 : L1) * ((q & 1) ? iL2 : iO2);
@@ -319,7 +367,7 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
                         const int jb = part * per, je = min(nj, jb + per);
                         double t00 = 0.0, t01 = 0.0, t10 = 0.0, t11 = 0.0;
                         for (int jj = jb; jj < je; ++jj) {
-                            const double z21 = zs[(2 * jj) * npad + b], z31 = zs[(2 * jj + 1) * npad + b];
+                            const double z21 = zs[(2 * jj) * ldz + b], z31 = zs[(2 * jj + 1) * ldz + b];
                             const double wj = sw2[j0 + jj], wdj = swd2[j0 + jj];
                             t00 = fma(wj, z21, t00);
                             t01 = fma(wdj, z21, t01);
@@ -348,44 +396,72 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
                         unsafeAtomicAdd(a.D + (int64_t)A * NB + A, s00);
                     }
                 }
-                // rank-kc update of the wave's blocks q < nbw: lane l supplies A[l%16][k] =
-                // Z[row][k] v[k] and B[k][l%16] = Z[col][k] for k = kk0 + l/16 (A = 0 past the
-                // chunk); operands of four blocks are loaded before their MFMAs
-                for (int kk0 = 0; kk0 < kc; kk0 += 4) {
+                // rank-kc update of the wave's blocks: lane l supplies A[l%16][k] = Z[row][k] v[k]
+                // and B[k][l%16] = Z[col][k] for k = kk0 + l/16 (A = 0 past the chunk); the
+                // operands of the next four rows are loaded before the MFMAs of these four
+                const unsigned char* zb = (const unsigned char*)zs;
+                auto row_of = [&](int kk0, double& v) -> const unsigned char* {
                     const int k = kk0 + (lane >> 4);
                     const bool kv = k < kc;
-                    const double v = kv ? vk[k] : 0.0;
-                    const double* zr = zs + (kv ? k : 0) * npad;
+                    v = kv ? vk[k] : 0.0;
+                    return zb + (uint32_t)((kv ? k : 0) * ldz) * 8u;
+                };
+                double an[NQ], bn[NQ], vn;
+                {
+                    const unsigned char* zr = row_of(0, vn);
 #pragma unroll
-                    for (int g = 0; g < BPW; g += 4) {
-                        if (g >= nbw) break;
-                        double av[4], bw[4];
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) {
-                            av[q] = zr[boa[g + q]];
-                            bw[q] = zr[bob[g + q]];
-                        }
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-                            if (g + q < nbw)
-                                acc[g + q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q] * v, bw[q], acc[g + q], 0, 0, 0);
+                    for (int q = 0; q < NQ; ++q) {
+                        an[q] = *(const double*)(zr + boa[q] * 8u);
+                        bn[q] = *(const double*)(zr + bob[q] * 8u);
                     }
+                }
+                for (int kk0 = 0; kk0 < kc; kk0 += 4) {
+                    double av[NQ], bw[NQ];
+                    const double v = vn;
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        av[q] = an[q];
+                        bw[q] = bn[q];
+                    }
+                    const unsigned char* zr = row_of(kk0 + 4, vn);
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        an[q] = *(const double*)(zr + boa[q] * 8u);
+                        bn[q] = *(const double*)(zr + bob[q] * 8u);
+                    }
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q)
+                        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q] * v, bw[q], acc[q], 0, 0, 0);
                 }
                 __syncthreads();
             }
             // upper triangle into D (act is ascending, so act[row] <= act[col] when row <= col);
             // accumulator element r of lane l is D_pair[l/16 + 4 r][l%16] of the block
 #pragma unroll
-            for (int q = 0; q < BPW; ++q) {
-                if (q >= nbw) continue;
-                const int col = bob[q];
+            for (int q = 0; q < NQ; ++q) {
+                if (!bval[q]) continue;
+                const int col = (int)bob[q];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int row = boa[q] - (lane & 15) + (lane >> 4) + 4 * r;
+                    const int row = (int)boa[q] - (lane & 15) + (lane >> 4) + 4 * r;
                     if (row <= col && col < nact)
                         unsafeAtomicAdd(a.D + (int64_t)act[row] * NB + act[col], acc[q][r]);
                 }
             }
+        };
+        for (int t0 = 0; t0 < nblk;) {
+            // blocks per wave in this pass: all waves take the same count
+            const int nq = min(BPW, (nblk - t0 + DWAVES - 1) / DWAVES);
+            switch (nq) {
+#define LYA_DIST_PASS(n) \
+    case n: pass(std::integral_constant<int, n>{}, t0); break;
+                LYA_DIST_PASS(1) LYA_DIST_PASS(2) LYA_DIST_PASS(3) LYA_DIST_PASS(4)
+                LYA_DIST_PASS(5) LYA_DIST_PASS(6) LYA_DIST_PASS(7) LYA_DIST_PASS(8)
+                LYA_DIST_PASS(9) LYA_DIST_PASS(10)
+#undef LYA_DIST_PASS
+                default: break;
+            }
+            t0 += DWAVES * nq;
         }
     }
 }
@@ -447,11 +523,11 @@ int distortion_launch(int32_t n_forest, const int64_t* pix_off, const double* dc
     const size_t fixed = lds_fixed(NB, max_len) + LDS_STATIC;
     const int64_t npad_max = (int64_t)(NB + BS - 1) / BS * BS;
     const size_t budget = prop.sharedMemPerBlock;
-    if (fixed + (size_t)16 * BS > budget)
+    if (fixed + (size_t)16 * (BS + ZSKEW) > budget)
         return lya_fail(LYA_ERR_UNSUPPORTED, "distortion: %d bins and forests of %d pixels exceed the LDS", NB,
                         max_len);
     const int64_t zcap = (int64_t)((budget - fixed) / 8);
-    const int32_t zc = (int32_t)std::min<int64_t>(zcap, (int64_t)(KC_MAX + 4) * npad_max);
+    const int32_t zc = (int32_t)std::min<int64_t>(zcap, (int64_t)(KC_MAX + 4) * (npad_max + ZSKEW));
     const size_t lds = lds_fixed(NB, max_len) + (size_t)8 * zc;
     if (hipFuncSetAttribute((const void*)distortion_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
         hipSuccess)
