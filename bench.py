@@ -325,9 +325,10 @@ def main():
         work = None if mode == "fp64-walk" else eng.work_stats(xi.default_config(base, **shape))
         out[mode] = {"wall": wall, "ev": ev, "res": res, "shape": shape, "with_counts_s": t_cnt, "work": work,
                      "binned": float(res[0].sum())}
-        log("%s %s: %.4f s/step (events median %.4f s) -> %.4g pixel-pairs/s; binned %.4g, evaluated %s"
+        log("%s %s: %.4f s/step (events median %.4f s) -> %.4g pixel-pairs/s; binned %.4g, evaluated %s%s"
             % (mode, shape, wall, float(np.median(ev)), P / wall, out[mode]["binned"],
-               "%.4g" % work["evaluated_pairs"] if work else "n/a"))
+               "%.4g" % work["evaluated_pairs"] if work else "n/a",
+               "; tasks %d, non-empty %d" % (work["tasks"], work["tasks_nonempty"]) if work else ""))
 
     if args.copy_calibrate:
         from lya2pcf_amd import _lib
@@ -375,6 +376,8 @@ def main():
                         "with_counts_ms": None if v["with_counts_s"] is None else v["with_counts_s"] * 1e3,
                         "binned_pairs": v["binned"],
                         "evaluated_pairs": v["work"]["evaluated_pairs"] if v["work"] else None,
+                        "tasks": v["work"]["tasks"] if v["work"] else None,
+                        "tasks_nonempty": v["work"]["tasks_nonempty"] if v["work"] else None,
                         "frac_binned": v["binned"] * FLOP_PER_PAIR / float(np.median(v["ev"])) / 1e12 /
                         (FP64_PEAK_TFLOPS if k.startswith("fp64") else FP32_PEAK_TFLOPS)}
                     for k, v in out.items()},
