@@ -202,3 +202,43 @@ def test_tile_group_counts(small_set, small_pairs, xcd_groups):
         eng.run(xi.default_config(mode, **shape))
         c, _, _ = eng.results()
         assert np.array_equal(c, c0), (xcd_groups, mode, shape)
+
+
+def _shifted_set(n, seed):
+    """Forests of 50-600 pixels (the 300k-forest recipe at small size) with every forest's
+    distances shifted by a seeded U(-600, 600) Mpc/h: many forest pairs have no pixel pair
+    within rmax, and many pairs only over part of forest 1, so the trimmed ranges start
+    and end inside the forests with every remainder modulo 64."""
+    s = synth.generate(synth.scaled(synth.CONFIGS["sky300k"], n, seed=seed))
+    fs = forests.ForestSet.from_survey(s, float32_inputs=True)
+    rng = np.random.default_rng(seed)
+    shift = rng.uniform(-600.0, 600.0, fs.nf)
+    dc = fs.dc.copy()
+    for f in range(fs.nf):
+        dc[fs.pix_off[f]:fs.pix_off[f + 1]] += shift[f]
+    return forests.ForestSet(fs.pix_off, fs.slots, **{**{k: getattr(fs, k) for k in fs.FOREST + fs.PIXEL},
+                                                      "dc": dc})
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_trimmed_units_and_tails(seed):
+    """Each forest pair is trimmed to the forest-1 pixels that can pair before its slabs
+    are cut, and the tail is walked against the forest-2 pixels that can pair with it:
+    every tail threshold (off, 16, 32, 48, 63 pixels), both slab widths, both precisions,
+    merged runs on and off give the oracle's counts bit for bit and its sums to 1e-12."""
+    fs = _shifted_set(150, seed)
+    am = forests.angmax_from(fs, 200.0)
+    off, idx = xi.build_pairs(fs, am)
+    c0, w0, d0 = oracle.xi_gpu_form(fs, off, idx)
+    aw, ad = _abs_sums(fs, off, idx)
+    print("seed", seed, "forest pairs", len(idx), "pixel pairs binned", int(c0.sum()), flush=True)
+    eng = xi.XiEngine(fs, am, pairs=(off, idx))
+    stats = eng.work_stats(xi.default_config("fp64", tail_max=-1))
+    assert stats["tasks"] > 0
+    for mode, ppl, merge, tail in itertools.product(["fp64", "mixed"], [1, 2], [0, 1], [-1, 16, 32, 48, 63]):
+        eng.run(xi.default_config(mode, px_per_lane=ppl, merge=merge, tail_max=tail))
+        c, w, d = eng.results()
+        label = (mode, ppl, merge, tail)
+        assert np.array_equal(c, c0), "%s: counts differ in %d bins" % (label, int((c != c0).sum()))
+        assert np.all(np.abs(w - w0) <= 1e-12 * aw + 1e-300), label
+        assert np.all(np.abs(d - d0) <= 1e-12 * ad + 1e-300), label
