@@ -1303,27 +1303,29 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
             const int nb1 = cptr(p.items)[4 * it + 2];
             const int64_t p1 = cptr(p.pix_off)[f1];
             const int n1 = (int)(cptr(p.pix_off)[f1 + 1] - p1);
-            const int nslab = (n1 + SL - 1) / SL;
-            const int ntask = nslab * (nb1 - nb0);
-            for (int k = wave; k < ntask; k += WAVES) {
-                const int q = k / nslab;
-                const int slab = k - q * nslab;
-                const int f2 = cptr(p.nb_idx)[nb0 + q];
+            // one neighbour per wave: the pair's geometry once, forest 1 trimmed to the
+            // pixels [ia, ib) that can pair with it (as in the forward kernel), then its slabs
+            for (int q = nb0 + wave; q < nb1; q += WAVES) {
+                const int f2 = cptr(p.nb_idx)[q];
                 const int64_t p2 = cptr(p.pix_off)[f2];
                 const int n2 = (int)(cptr(p.pix_off)[f2 + 1] - p2);
+                if (n1 <= 0 || n2 <= 0) continue;
                 const PairGeo g = pair_geo(p, f1, f2, bp64, bt64);
                 const double c = g.c, s = g.s;
-                const int ilast = min(n1, slab * SL + SL) - 1;
+                int ia, ib;
+                prune(p, f2, p2, 0, n2 - 1, f1, p1, 0, n1, g, lane, ia, ib);
+                for (int i0 = ia; i0 < ib; i0 += SL) {
+                const int ilast = min(ib, i0 + SL) - 1;
                 int jlo, jhi;
-                prune(p, f1, p1, slab * SL, ilast, f2, p2, 0, n2, g, lane, jlo, jhi);
+                prune(p, f1, p1, i0, ilast, f2, p2, 0, n2, g, lane, jlo, jhi);
                 if (jlo >= jhi) continue;
                 int i[PPL];
                 bool valid[PPL];
                 double r1[PPL], gd[PPL], gw[PPL];
 #pragma unroll
                 for (int m = 0; m < PPL; ++m) {
-                    i[m] = slab * SL + m * 64 + lane;
-                    valid[m] = i[m] < n1;
+                    i[m] = i0 + m * 64 + lane;
+                    valid[m] = i[m] < ib;
                     r1[m] = valid[m] ? p.dc[p1 + i[m]] : 1.0e9;   // far away: no pair in range
                     gd[m] = 0.0;
                     gw[m] = 0.0;
@@ -1369,7 +1371,7 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
                     };
                     int j = jlo;
                     for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
-                    for (; j < jhi; ++j) block(j, std::integral_constant<int, 1>{});
+                    rest_blocks<U>(block, j, jhi);
                 } else {
                     const double base2 = cptr(p.base)[f2];
                     const double cb = c * bp64, sb = s * bt64;
@@ -1438,7 +1440,7 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
                     };
                     int j = jlo;
                     for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
-                    for (; j < jhi; ++j) block(j, std::integral_constant<int, 1>{});
+                    rest_blocks<U>(block, j, jhi);
                 }
 #pragma unroll
                 for (int m = 0; m < PPL; ++m) {
@@ -1446,6 +1448,7 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
                         unsafeAtomicAdd(g_dw + p1 + i[m], gd[m]);
                         unsafeAtomicAdd(g_w + p1 + i[m], gw[m]);
                     }
+                }
                 }
             }
         }
