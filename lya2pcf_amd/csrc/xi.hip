@@ -1315,140 +1315,140 @@ __global__ void __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu
                 int ia, ib;
                 prune(p, f2, p2, 0, n2 - 1, f1, p1, 0, n1, g, lane, ia, ib);
                 for (int i0 = ia; i0 < ib; i0 += SL) {
-                const int ilast = min(ib, i0 + SL) - 1;
-                int jlo, jhi;
-                prune(p, f1, p1, i0, ilast, f2, p2, 0, n2, g, lane, jlo, jhi);
-                if (jlo >= jhi) continue;
-                int i[PPL];
-                bool valid[PPL];
-                double r1[PPL], gd[PPL], gw[PPL];
-#pragma unroll
-                for (int m = 0; m < PPL; ++m) {
-                    i[m] = i0 + m * 64 + lane;
-                    valid[m] = i[m] < ib;
-                    r1[m] = valid[m] ? p.dc[p1 + i[m]] : 1.0e9;   // far away: no pair in range
-                    gd[m] = 0.0;
-                    gw[m] = 0.0;
-                }
-                // in-range pairs gather alpha, beta of their bin; others read bin 0 times zero
-                auto gather = [&](int m, uint32_t bp, uint32_t bt, double dwj, double wj) {
-                    const bool in = bp < (uint32_t)NP && bt < (uint32_t)NT;
-                    double2 ab;
-                    if constexpr (GTAB) {
-                        const uint32_t A = in ? __umul24(bp, (uint32_t)NT) + bt : 0u;
-                        ab = make_double2(in ? alpha[A] : 0.0, in ? beta[A] : 0.0);
-                    } else {
-                        ab = tab[in ? __umul24(bp, (uint32_t)NT) + bt : (uint32_t)nb];
-                    }
-                    gd[m] = fma(ab.x, dwj, gd[m]);
-                    gw[m] = fma(ab.y, wj, gw[m]);
-                };
-                // forest-2 pixels in blocks of U wave-uniform scalar loads, as in the forward walk
-                constexpr int U = 8;
-                const CAS double* d2p = cptr(p.dw) + p2;
-                const CAS double* w2p = cptr(p.we) + p2;
-                if constexpr (MODE == LYA_MODE_FP64) {
-                    const CAS double* r2p = cptr(p.dc) + p2;
-                    const double cq = POW2 ? c * bp64 : c, sq = POW2 ? s * bt64 : s;
-                    auto block = [&](int j, auto UC) {
-                        constexpr int UU = decltype(UC)::value;
-                        double r2[UU], d2[UU], w2[UU];
-#pragma unroll
-                        for (int u = 0; u < UU; ++u) {
-                            r2[u] = r2p[j + u];
-                            d2[u] = d2p[j + u];
-                            w2[u] = w2p[j + u];
-                        }
-#pragma unroll
-                        for (int u = 0; u < UU; ++u) {
-#pragma unroll
-                            for (int m = 0; m < PPL; ++m) {
-                                const double qp = POW2 ? fabs(r1[m] - r2[u]) * cq : (fabs(r1[m] - r2[u]) * c) * bp64;
-                                const double qt = POW2 ? (r1[m] + r2[u]) * sq : ((r1[m] + r2[u]) * s) * bt64;
-                                gather(m, (uint32_t)__double2int_rz(qp), (uint32_t)__double2int_rz(qt), d2[u], w2[u]);
-                            }
-                        }
-                    };
-                    int j = jlo;
-                    for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
-                    rest_blocks<U>(block, j, jhi);
-                } else {
-                    const double base2 = cptr(p.base)[f2];
-                    const double cb = c * bp64, sb = s * bt64;
-                    const float c4 = unif((float)cb), s4 = unif((float)sb);
-                    const float tau = p.tau, tau2 = 2.0f * tau;
-                    // lanes past the end of forest 1: 2^20 + 1/2 with zero slopes (fraction
-                    // 1/2, out of range, never rechecked)
-                    float P1[PPL], T1[PPL], ncl[PPL], scl[PPL];
+                    const int ilast = min(ib, i0 + SL) - 1;
+                    int jlo, jhi;
+                    prune(p, f1, p1, i0, ilast, f2, p2, 0, n2, g, lane, jlo, jhi);
+                    if (jlo >= jhi) continue;
+                    int i[PPL];
+                    bool valid[PPL];
+                    double r1[PPL], gd[PPL], gw[PPL];
 #pragma unroll
                     for (int m = 0; m < PPL; ++m) {
-                        P1[m] = valid[m] ? (float)((r1[m] - base2) * cb) : 1048576.5f;
-                        T1[m] = valid[m] ? (float)((r1[m] + base2) * sb + (double)tau) : 1048576.5f;
-                        ncl[m] = valid[m] ? -c4 : 0.0f;
-                        scl[m] = valid[m] ? s4 : 0.0f;
+                        i[m] = i0 + m * 64 + lane;
+                        valid[m] = i[m] < ib;
+                        r1[m] = valid[m] ? p.dc[p1 + i[m]] : 1.0e9;   // far away: no pair in range
+                        gd[m] = 0.0;
+                        gw[m] = 0.0;
                     }
-                    const CAS float* o2p = cptr(p.off32) + p2;
-                    const CAS double* r2p = cptr(p.dc) + p2;
-                    auto block = [&](int j, auto UC) {
-                        constexpr int UU = decltype(UC)::value;
-                        float o2[UU];
-                        double d2[UU], w2[UU];
-                        uint32_t bp[UU][PPL], bt[UU][PPL];
-                        float frmin = 1.0f;
-#pragma unroll
-                        for (int u = 0; u < UU; ++u) {
-                            o2[u] = o2p[j + u];
-                            d2[u] = d2p[j + u];
-                            w2[u] = w2p[j + u];
+                    // in-range pairs gather alpha, beta of their bin; others read bin 0 times zero
+                    auto gather = [&](int m, uint32_t bp, uint32_t bt, double dwj, double wj) {
+                        const bool in = bp < (uint32_t)NP && bt < (uint32_t)NT;
+                        double2 ab;
+                        if constexpr (GTAB) {
+                            const uint32_t A = in ? __umul24(bp, (uint32_t)NT) + bt : 0u;
+                            ab = make_double2(in ? alpha[A] : 0.0, in ? beta[A] : 0.0);
+                        } else {
+                            ab = tab[in ? __umul24(bp, (uint32_t)NT) + bt : (uint32_t)nb];
                         }
-                        auto qpt = [&](int u, int m, float& qp, float& qt) {
-                            // r_par below half a bin: bin 0 either way, kept out of the recheck
-                            qp = fmaxf(__builtin_fabsf(__builtin_fmaf(o2[u], ncl[m], P1[m])) + tau, 0.5f);
-                            qt = __builtin_fmaf(o2[u], scl[m], T1[m]);
-                        };
+                        gd[m] = fma(ab.x, dwj, gd[m]);
+                        gw[m] = fma(ab.y, wj, gw[m]);
+                    };
+                    // forest-2 pixels in blocks of U wave-uniform scalar loads, as in the forward walk
+                    constexpr int U = 8;
+                    const CAS double* d2p = cptr(p.dw) + p2;
+                    const CAS double* w2p = cptr(p.we) + p2;
+                    if constexpr (MODE == LYA_MODE_FP64) {
+                        const CAS double* r2p = cptr(p.dc) + p2;
+                        const double cq = POW2 ? c * bp64 : c, sq = POW2 ? s * bt64 : s;
+                        auto block = [&](int j, auto UC) {
+                            constexpr int UU = decltype(UC)::value;
+                            double r2[UU], d2[UU], w2[UU];
 #pragma unroll
-                        for (int u = 0; u < UU; ++u) {
-#pragma unroll
-                            for (int m = 0; m < PPL; ++m) {
-                                float qp, qt;
-                                qpt(u, m, qp, qt);
-                                bp[u][m] = __float2uint_rz(qp);
-                                bt[u][m] = __float2uint_rz(qt);
-                                frmin = fminf(frmin, fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt)));
+                            for (int u = 0; u < UU; ++u) {
+                                r2[u] = r2p[j + u];
+                                d2[u] = d2p[j + u];
+                                w2[u] = w2p[j + u];
                             }
+#pragma unroll
+                            for (int u = 0; u < UU; ++u) {
+#pragma unroll
+                                for (int m = 0; m < PPL; ++m) {
+                                    const double qp = POW2 ? fabs(r1[m] - r2[u]) * cq : (fabs(r1[m] - r2[u]) * c) * bp64;
+                                    const double qt = POW2 ? (r1[m] + r2[u]) * sq : ((r1[m] + r2[u]) * s) * bt64;
+                                    gather(m, (uint32_t)__double2int_rz(qp), (uint32_t)__double2int_rz(qt), d2[u], w2[u]);
+                                }
+                            }
+                        };
+                        int j = jlo;
+                        for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
+                        rest_blocks<U>(block, j, jhi);
+                    } else {
+                        const double base2 = cptr(p.base)[f2];
+                        const double cb = c * bp64, sb = s * bt64;
+                        const float c4 = unif((float)cb), s4 = unif((float)sb);
+                        const float tau = p.tau, tau2 = 2.0f * tau;
+                        // lanes past the end of forest 1: 2^20 + 1/2 with zero slopes (fraction
+                        // 1/2, out of range, never rechecked)
+                        float P1[PPL], T1[PPL], ncl[PPL], scl[PPL];
+#pragma unroll
+                        for (int m = 0; m < PPL; ++m) {
+                            P1[m] = valid[m] ? (float)((r1[m] - base2) * cb) : 1048576.5f;
+                            T1[m] = valid[m] ? (float)((r1[m] + base2) * sb + (double)tau) : 1048576.5f;
+                            ncl[m] = valid[m] ? -c4 : 0.0f;
+                            scl[m] = valid[m] ? s4 : 0.0f;
                         }
-                        // the edge test is recomputed here rather than kept per pair
-                        if (frmin < tau2) {
+                        const CAS float* o2p = cptr(p.off32) + p2;
+                        const CAS double* r2p = cptr(p.dc) + p2;
+                        auto block = [&](int j, auto UC) {
+                            constexpr int UU = decltype(UC)::value;
+                            float o2[UU];
+                            double d2[UU], w2[UU];
+                            uint32_t bp[UU][PPL], bt[UU][PPL];
+                            float frmin = 1.0f;
+#pragma unroll
+                            for (int u = 0; u < UU; ++u) {
+                                o2[u] = o2p[j + u];
+                                d2[u] = d2p[j + u];
+                                w2[u] = w2p[j + u];
+                            }
+                            auto qpt = [&](int u, int m, float& qp, float& qt) {
+                                // r_par below half a bin: bin 0 either way, kept out of the recheck
+                                qp = fmaxf(__builtin_fabsf(__builtin_fmaf(o2[u], ncl[m], P1[m])) + tau, 0.5f);
+                                qt = __builtin_fmaf(o2[u], scl[m], T1[m]);
+                            };
 #pragma unroll
                             for (int u = 0; u < UU; ++u) {
 #pragma unroll
                                 for (int m = 0; m < PPL; ++m) {
                                     float qp, qt;
                                     qpt(u, m, qp, qt);
-                                    if (fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt)) < tau2) {
-                                        const double r2 = r2p[j + u];
-                                        bp[u][m] = (uint32_t)__double2int_rz((fabs(r1[m] - r2) * c) * bp64);
-                                        bt[u][m] = (uint32_t)__double2int_rz(((r1[m] + r2) * s) * bt64);
+                                    bp[u][m] = __float2uint_rz(qp);
+                                    bt[u][m] = __float2uint_rz(qt);
+                                    frmin = fminf(frmin, fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt)));
+                                }
+                            }
+                            // the edge test is recomputed here rather than kept per pair
+                            if (frmin < tau2) {
+#pragma unroll
+                                for (int u = 0; u < UU; ++u) {
+#pragma unroll
+                                    for (int m = 0; m < PPL; ++m) {
+                                        float qp, qt;
+                                        qpt(u, m, qp, qt);
+                                        if (fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt)) < tau2) {
+                                            const double r2 = r2p[j + u];
+                                            bp[u][m] = (uint32_t)__double2int_rz((fabs(r1[m] - r2) * c) * bp64);
+                                            bt[u][m] = (uint32_t)__double2int_rz(((r1[m] + r2) * s) * bt64);
+                                        }
                                     }
                                 }
                             }
-                        }
 #pragma unroll
-                        for (int u = 0; u < UU; ++u)
+                            for (int u = 0; u < UU; ++u)
 #pragma unroll
-                            for (int m = 0; m < PPL; ++m) gather(m, bp[u][m], bt[u][m], d2[u], w2[u]);
-                    };
-                    int j = jlo;
-                    for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
-                    rest_blocks<U>(block, j, jhi);
-                }
-#pragma unroll
-                for (int m = 0; m < PPL; ++m) {
-                    if (valid[m]) {
-                        unsafeAtomicAdd(g_dw + p1 + i[m], gd[m]);
-                        unsafeAtomicAdd(g_w + p1 + i[m], gw[m]);
+                                for (int m = 0; m < PPL; ++m) gather(m, bp[u][m], bt[u][m], d2[u], w2[u]);
+                        };
+                        int j = jlo;
+                        for (; j + U <= jhi; j += U) block(j, std::integral_constant<int, U>{});
+                        rest_blocks<U>(block, j, jhi);
                     }
-                }
+#pragma unroll
+                    for (int m = 0; m < PPL; ++m) {
+                        if (valid[m]) {
+                            unsafeAtomicAdd(g_dw + p1 + i[m], gd[m]);
+                            unsafeAtomicAdd(g_w + p1 + i[m], gw[m]);
+                        }
+                    }
                 }
             }
         }
