@@ -555,16 +555,21 @@ __device__ __forceinline__ void run_task(const lya_xi_problem& p, unsigned char*
                         float qp, qt;
                         qpt(u, k, qp, qt);
                         key[u][k] = bin_key(__float2uint_rz(qp), __float2uint_rz(qt));
-                        frmin = fminf(frmin, fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt)));
+                        frmin = fminf(fminf(frmin, __builtin_amdgcn_fractf(qp)), __builtin_amdgcn_fractf(qt));
                     }
                 }
                 if (frmin < tau2) {
 #pragma unroll
                     for (int u = 0; u < UU; ++u) {
+                        // an opaque copy of the forest-2 offset: the recheck recomputes the
+                        // fractions, so the main loop keeps none per pair and folds them into
+                        // frmin with one v_min3 per pair
+                        float ou = o2v[u];
+                        asm volatile("" : "+v"(ou));
 #pragma unroll
                         for (int k = 0; k < PPL; ++k) {
-                            float qp, qt;
-                            qpt(u, k, qp, qt);
+                            const float qp = fmaxf(__builtin_fabsf(__builtin_fmaf(ou, ncl[k], P1[k])) + tau, 0.5f);
+                            const float qt = __builtin_fmaf(ou, scl[k], T1[k]);
                             if (fminf(__builtin_amdgcn_fractf(qp), __builtin_amdgcn_fractf(qt)) < tau2) {
                                 const double r2 = r2p[jv + u];
                                 key[u][k] = bin_key((uint32_t)__double2int_rz((fabs(r1[k] - r2) * c) * bp64),
