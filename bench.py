@@ -362,6 +362,9 @@ def main():
                      # workload only; other workloads report none
                      "traffic": autotune.measured_traffic(main_mode) if args.config == "sky100k" and
                      not args.forests else None,
+                     # VALU issue and LDS figures of the same kernel and workload (rocprofv3 counters)
+                     "counters": autotune.measured_counters(main_mode) if args.config == "sky100k" and
+                     not args.forests else None,
                      "pixel_pairs": P, "binned_pairs": B,
                      "evaluated_pairs": m["work"]["evaluated_pairs"] if m["work"] else None,
                      "binned_per_pixel_pair": B / P,
@@ -383,6 +386,9 @@ def main():
                     for k, v in out.items()},
         "setup": setup,
     }
+    if line["roofline"]["traffic"]:
+        # achieved HBM rate: the measured bytes per launch over this run's launch time
+        line["roofline"]["hbm_gbps"] = line["roofline"]["traffic"] / ev_med / 1e9
     fp64s = [k for k in ("fp64", "fp64-walk") if k in out]
     if fp64s and "mixed" in out:
         best64 = min(fp64s, key=lambda k: out[k]["wall"])

@@ -70,6 +70,20 @@ def measured_traffic(mode="mixed"):
         return None
 
 
+def measured_counters(mode="mixed"):
+    """VALU issue, LDS bank-conflict and wait fractions of the tile kernel from the
+    committed rocprofv3 counter summary, if any."""
+    path = os.path.join(PROFILES, "counters_xi_%s_sky100k.json" % mode)
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    keys = ("valu_issue", "lds_bank_conflict_per_cu_cycle", "lds_wait_per_wave_cycle",
+            "valu_per_evaluated_wave_step", "cycles_per_xcd", "source")
+    return {k: d[k] for k in keys if k in d}
+
+
 def candidates(mode, full=False):
     """Shapes to time: the tile space (default or full) and, for fp64, the walk."""
     spaces = [FULL_SPACE if full else SPACE] + ([WALK_SPACE] if mode == "fp64" else [])
