@@ -111,7 +111,7 @@ __host__ __device__ constexpr size_t lds_fixed(int nb, int max_len) {
 
 __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ int s_pair, s_nact;
+    __shared__ int s_pair, s_nact, s_wcnt[DWAVES];
     const int NB = a.np_rp * a.np_rt;
     const int NP = a.np_rp, NT = a.np_rt;
     int16_t* cmap = (int16_t*)lds;                                     // [NB] bin -> compact, -1
@@ -158,11 +158,17 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
         }
     };
 
+    // the queue index of the next pair is drawn while this one runs and published at its end
+    if (tid == 0) s_pair = (int)atomicAdd(a.queue, 1u);
     for (;;) {
-        if (tid == 0) s_pair = (int)atomicAdd(a.queue, 1u);
         __syncthreads();
         const int pr = uni(s_pair);
         if (pr >= a.n_pairs) break;
+        int nxt = 0;
+        if (tid == 0) nxt = (int)atomicAdd(a.queue, 1u);
+        auto publish = [&]() {
+            if (tid == 0) s_pair = nxt;
+        };
         const int f1 = cptr(a.pairs)[2 * pr], f2 = cptr(a.pairs)[2 * pr + 1];
         const int64_t p1 = cptr(a.pix_off)[f1], p2 = cptr(a.pix_off)[f2];
         const int n1 = (int)(cptr(a.pix_off)[f1 + 1] - p1), n2 = (int)(cptr(a.pix_off)[f2 + 1] - p2);
@@ -171,6 +177,7 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
             // keeps thread 0 from drawing the next pair before every wave has read this one
             if (tid == 0) atomicAdd(a.skip, 1u);
             __syncthreads();
+            publish();
             continue;
         }
         const double O1 = cptr(a.omega)[f1], O2 = cptr(a.omega)[f2];
@@ -210,10 +217,13 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
             if (hb && gb >= 0) cmap[gb] = 1;
         });
         __syncthreads();
-        // compaction in bin order (wave 0)
-        if (wave == 0) {
-            const int per = (NB + 63) / 64;
-            const int b0 = lane * per, b1 = min(NB, b0 + per);
+        // compaction in bin order: wave w numbers the bins of its eighth of the binning,
+        // after the counts of the waves before it
+        {
+            const int perw = (NB + DWAVES - 1) / DWAVES;
+            const int wb0 = min(NB, wave * perw), wb1 = min(NB, wb0 + perw);
+            const int per = (perw + 63) / 64;
+            const int b0 = min(wb1, wb0 + lane * per), b1 = min(wb1, b0 + per);
             int cnt = 0;
             for (int b = b0; b < b1; ++b) cnt += (cmap[b] >= 0);
             int incl = cnt;
@@ -221,7 +231,11 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
                 const int v = __shfl_up(incl, o);
                 if (lane >= o) incl += v;
             }
-            int idx = incl - cnt;
+            if (lane == 63) s_wcnt[wave] = incl;
+            __syncthreads();
+            int base = 0;
+            for (int w = 0; w < wave; ++w) base += s_wcnt[w];
+            int idx = base + incl - cnt;
             for (int b = b0; b < b1; ++b) {
                 if (cmap[b] >= 0) {
                     cmap[b] = (int16_t)idx;
@@ -229,16 +243,20 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
                     ++idx;
                 }
             }
-            if (lane == 63) s_nact = incl;
+            if (tid == DTHREADS - 1) s_nact = base + incl;
         }
         __syncthreads();
         const int nact = uni(s_nact);
-        if (nact == 0) continue;
+        if (nact == 0) {
+            publish();
+            continue;
+        }
         // the LDS chunk buffer must hold two Z rows of the pair's active bins (padded to
         // the block edge, plus the row skew); a pair with more active bins is skipped and
         // counted in queue[2]
         if (a.zcap / (((nact + BS - 1) / BS) * BS + ZSKEW) < 2) {
             if (tid == 0) atomicAdd(a.skip + 1, 1u);
+            publish();
             continue;
         }
         // compact bin of pixel pair (i, j), or -1
@@ -463,6 +481,7 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
             }
             t0 += DWAVES * nq;
         }
+        publish();
     }
 }
 
