@@ -322,7 +322,9 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
                         vk[tid] = (tid & 1) ? -wj * L1 : -wj * O1;
                     }
                     __syncthreads();
-                    grid2(nj, n1, [&](int ja, int ia, int jb, int ib, bool hb) {
+                    // consecutive threads on consecutive rows (forest-2 pixels) of the chunk: runs
+                    // of equal bins along forest 1 would put neighbouring lanes on one address
+                    grid2(n1, nj, [&](int ia, int ja, int ib, int jb, bool hb) {
                         const int ba = cbin(ia, j0 + ja), bb = cbin(ib, j0 + jb);
                         if (ba >= 0) {
                             atomicAdd(zs + (2 * ja) * ldz + ba, sw1[ia] * iO1);
