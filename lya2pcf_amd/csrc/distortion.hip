@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(DTHREADS) distortion_kernel(const DArgs a) {
         const int n1 = (int)(cptr(a.pix_off)[f1 + 1] - p1), n2 = (int)(cptr(a.pix_off)[f2 + 1] - p2);
         if (n1 > a.max_len || n2 > a.max_len) {
             // longer than the staging arrays: skipped, and counted in queue[1]; the barrier
-            // keeps thread 0 from drawing the next pair before every wave has read this one
+            // keeps thread 0 from publishing the next pair before every wave has read this one
             if (tid == 0) atomicAdd(a.skip, 1u);
             __syncthreads();
             publish();
