@@ -24,3 +24,23 @@ def test_default_space_inside_full_space():
     walks = [c for c in autotune.candidates("fp64") if c.get("walk")]
     assert {c["waves"] for c in walks} == {8, 16}
     assert not any(c.get("walk") for c in autotune.candidates("mixed", full=True))
+
+
+def test_measured_counters_of_the_committed_profiles():
+    # the bench line copies these figures; they must match the profile summaries' own
+    # arithmetic (VALU issue = VALU instructions / (256 CUs x cycles per XCD))
+    import json
+    import os
+    for mode in ("mixed", "fp64"):
+        c = autotune.measured_counters(mode)
+        assert c is not None and set(c) == {"valu_issue", "lds_bank_conflict_per_cu_cycle",
+                                            "lds_wait_per_wave_cycle", "valu_per_evaluated_wave_step",
+                                            "cycles_per_xcd", "source"}
+        with open(os.path.join(autotune.PROFILES, "counters_xi_%s_sky100k.json" % mode)) as f:
+            raw = json.load(f)["raw"]
+        cyc = raw["GRBM_GUI_ACTIVE"] / 8
+        assert abs(c["cycles_per_xcd"] - cyc) < 1e-6 * cyc
+        assert abs(c["valu_issue"] - raw["SQ_INSTS_VALU"] / (256 * cyc)) < 1e-9
+        assert abs(c["lds_wait_per_wave_cycle"] - raw["SQ_WAIT_INST_LDS"] / raw["SQ_WAVE_CYCLES"]) < 1e-9
+        assert 0.5 < c["valu_issue"] < 1.01
+    assert autotune.measured_counters("no-such-mode") is None
